@@ -1,0 +1,159 @@
+/*
+ * nf3d.h -- C ABI of the MI355X (gfx950) neo_fft3d denoiser.
+ *
+ * Drop-in boundary.  In the reference the per-plane engine FFT3DEngine
+ * (src/fft3d_engine.h:22-741) and the FFTW library behind it
+ * (src/fftwlite.h:21-96) sit under the FFT3D front end (src/fft3d.hpp:12-364),
+ * which the DualSynth bridge exposes to AviSynth+ as neo_fft3d() and to
+ * VapourSynth as neo_fft3d.FFT3D (include/dualsynth/avs_wrapper.hpp:211-228,
+ * vs_wrapper.hpp:195-212).  This library replaces the engine and FFTW with HIP
+ * kernels and also offers the front end itself, so a plugin binds either level:
+ *
+ *   engine level   nf3d_engine_*   one engine per processed plane, exactly
+ *                                  where FFT3DEngine is constructed and called
+ *   front-end      nf3d_filter_*   parameter parsing, defaults, plane
+ *                                  selection, crop copy, frame-request window
+ *
+ * Conventions: functions return 0 on success and -1 on failure with a
+ * NUL-terminated message in err (the reference's own wording for the errors
+ * it raises; the caller prefixes "FFT3D: " / "neo_fft3d: " as the bridge
+ * does).  Host pointers are borrowed for the duration of a call; device
+ * memory is owned by the library.  Plain C types only.
+ */
+#ifndef NF3D_H
+#define NF3D_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Every knob of EngineParams (src/fft3d_common.h:48-77), as the user gave it
+ * (bit-depth scaling of sigma/smin/smax happens inside, as in
+ * src/fft3d_engine.h:91-103). */
+typedef struct nf3d_params {
+  float sigma, beta;
+  int bw, bh, bt, ow, oh;
+  float kratio, sharpen, scutoff, svr, smin, smax;
+  int measure, interlaced, wintype, pframe, px, py, pshow;
+  float pcutoff, pfactor, sigma2, sigma3, sigma4, degrid, dehalo, hr, ht;
+  int l, t, r, b, opt;
+} nf3d_params;
+
+/* Clip format, as DSFormat (include/dualsynth/ds_format.hpp:10-26). */
+typedef struct nf3d_format {
+  int color_family;    /* 0 gray, 1 yuv, 2 rgb */
+  int is_float;        /* 32-bit float samples */
+  int bits_per_sample; /* 8, 9..16 (uint16 storage), 32 */
+  int ssw, ssh;        /* chroma subsampling shifts (yuv) */
+  int num_planes;      /* 1 or 3 */
+} nf3d_format;
+
+/* One plane of one frame.  `data` is host memory for the *_get_frame calls
+ * and device memory for the *_device calls; stride in bytes. */
+typedef struct nf3d_frame_ref {
+  int n;
+  const void* data;
+  int stride;
+} nf3d_frame_ref;
+
+typedef struct nf3d_engine nf3d_engine;
+
+/* Front-end defaults (src/fft3d.hpp:77-106): sigma2..sigma4 = sigma. */
+void nf3d_params_defaults(nf3d_params* p, float sigma);
+
+/* Replaces `new FFT3DEngine(ep, plane, ...)` (src/fft3d.hpp:186-198,
+ * src/fft3d_engine.h:87-407): validation, geometry, windows and tables, and
+ * the engine's device state on the current HIP device.  width/height are the
+ * clip's (luma) size; plane selects luma or chroma geometry. */
+int nf3d_engine_create(const nf3d_params* p, const nf3d_format* fmt, int width, int height,
+                       int frames, int plane, nf3d_engine** out, char* err, size_t errlen);
+
+/* Replaces FFT3DEngine::GetFrame(n, in_frames) (src/fft3d_engine.h:452-734).
+ * `frames` must hold this plane of every frame in the request window
+ * [max(n - bt/2, 0), min(n + (bt-1)/2, frames-1)] and, while the noise
+ * pattern is still to be estimated (pfactor != 0), of frame pframe.
+ * The filtered plane is written into dst's processed region (the plane minus
+ * the l/t/r/b crop); crop borders are left untouched.  Thread-safe: calls on
+ * one engine are serialised (bt == 0 is order dependent, as in the reference). */
+int nf3d_engine_get_frame(nf3d_engine* e, int n, const nf3d_frame_ref* frames, int nframes,
+                          void* dst, int dst_stride, char* err, size_t errlen);
+
+/* As nf3d_engine_get_frame, with device-resident planes: the work is
+ * enqueued on `stream` (a hipStream_t, or NULL for the engine's own stream)
+ * and the call returns without waiting.  dst may not alias a source plane. */
+int nf3d_engine_get_frame_device(nf3d_engine* e, int n, const nf3d_frame_ref* frames, int nframes,
+                                 void* dst, int dst_stride, void* stream, char* err, size_t errlen);
+
+/* Replaces FFT3DEngine::CacheRefresh (src/fft3d_engine.h:736-740): 1 if the
+ * spectrum of frame n is held in the HBM spectra ring (and marks it recent). */
+int nf3d_engine_cache_refresh(nf3d_engine* e, int n);
+
+/* Block geometry of the engine: nox, noy, outwidth, mirw, mirh, ow, oh. */
+int nf3d_engine_geometry(const nf3d_engine* e, int* out7);
+
+/* Wait for the engine's queued work (the host-fed calls already do). */
+int nf3d_engine_sync(nf3d_engine* e);
+
+void nf3d_engine_destroy(nf3d_engine* e);
+
+/* --------------------------------------------------------------- front end
+ * The FFT3D filter of src/fft3d.hpp: 41 parameters with the reference's
+ * defaults, `planes` (VapourSynth) or y/u/v (AviSynth+) plane modes, l/t/r/b
+ * crop with border copy, and the frame-request window. */
+typedef struct nf3d_filter nf3d_filter;
+
+/* Argument signatures generated by the bridge from FFT3D::Params()
+ * (include/dualsynth/ds_filter.hpp:19-69, src/fft3d.hpp:29-73). */
+const char* nf3d_vs_signature(void);
+const char* nf3d_avs_signature(void);
+
+/* Replaces FFT3D::Initialize (src/fft3d.hpp:74-199).  `args` is the
+ * invocation's arguments as "name=value;name=value" (integer lists
+ * comma-separated, e.g. "planes=0,2"); absent names take the defaults.
+ * Passing y/u/v instead of planes selects the AviSynth+ plane modes. */
+int nf3d_filter_create(const char* args, const nf3d_format* fmt, int width, int height, int frames,
+                       nf3d_filter** out, char* err, size_t errlen);
+
+/* FFT3D::RequestReferenceFrames (src/fft3d.hpp:201-219), plus frame pframe
+ * while the noise pattern is pending.  Returns the count (<= cap). */
+int nf3d_filter_request_frames(nf3d_filter* f, int n, int* out, int cap);
+
+/* 0: serial (bt == 0, fmSerial / MT_SERIALIZED), 1: parallel
+ * (fmParallel / MT_NICE_FILTER) -- src/fft3d.hpp:27-28. */
+int nf3d_filter_mt_mode(const nf3d_filter* f);
+
+/* One frame of the clip: all planes, host memory. */
+typedef struct nf3d_frame_planes {
+  int n;
+  const void* data[4];
+  int stride[4];
+} nf3d_frame_planes;
+
+/* FFT3D::GetFrame (src/fft3d.hpp:221-264): `frames` holds the frames named by
+ * nf3d_filter_request_frames(n); every plane of frame n is written to dst. */
+int nf3d_filter_get_frame(nf3d_filter* f, int n, const nf3d_frame_planes* frames, int nframes,
+                          void* const* dst, const int* dst_stride, char* err, size_t errlen);
+
+void nf3d_filter_destroy(nf3d_filter* f);
+
+/* ------------------------------------------------------------ clip-level
+ * A whole in-memory clip behind the front end, with the same signature as the
+ * oracle and reference test drivers (family 0 gray / 1 yuv / 2 rgb,
+ * sample_type 0 integer / 1 float, planes[frame * nplanes + plane]). */
+void* nf3d_clip_create(const char* args, int family, int sample_type, int bits, int ssw, int ssh,
+                       int width, int height, int nframes, const void* const* planes,
+                       const int* strides, char* err, int errlen);
+int nf3d_clip_get_frame(void* h, int n, void* const* dst, const int* dst_strides, char* err,
+                        int errlen);
+void nf3d_clip_free(void* h);
+
+/* Library identity: "Neo FFT3D Filter (MI355X/gfx950) <version>". */
+const char* nf3d_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NF3D_H */

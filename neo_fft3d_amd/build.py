@@ -1,0 +1,96 @@
+"""Builds the package's native libraries in-tree.
+
+  neo_fft3d_amd/libnf3d.so        the product: gfx950 kernels + C ABI (hipcc)
+  neo_fft3d_amd/libnf3d_synth.so  seeded synthetic clips (g++, host only)
+
+Run as ``python -m neo_fft3d_amd.build`` or through ``__graft_entry__.build()``.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+BUILD = os.path.join(ROOT, "_build")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("PYTORCH_ROCM_ARCH", "gfx950")
+# -ffp-contract=off: the spectral pass and the overlap-add follow the
+# reference's float rounding (the FFT opts into FMA explicitly);
+# -packed-fp32-ops: no v_pk_*_f32 generation.
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument", f"-I{os.path.join(ROOT, 'include')}",
+]
+
+KERNEL_UNITS = ["nf3d_kernels.hip"]
+HOST_UNITS = ["nf3d_engine.cpp", "nf3d_frontend.cpp"]
+
+
+def block_shapes():
+    """(bw, bh) pairs compiled for k_block, read from NF3D_SHAPES."""
+    import re
+    src = open(os.path.join(CSRC, "nf3d_kernels.hip")).read()
+    line = re.search(r"#define NF3D_SHAPES\(X\)(.*)", src).group(1)
+    return [(int(a), int(b)) for a, b in re.findall(r"X\((\d+),\s*(\d+)\)", line)]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{r.stdout}")
+    return r.stdout
+
+
+def build_product(verbose: bool = False) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    jobs = []
+    objs = []
+    for u in KERNEL_UNITS:
+        o = os.path.join(BUILD, u + ".o")
+        jobs.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, u), "-o", o])
+        objs.append(o)
+    for bw, bh in block_shapes():
+        o = os.path.join(BUILD, f"nf3d_shape_{bw}x{bh}.o")
+        jobs.append([HIPCC, *HIP_FLAGS, f"-DNF3D_BW={bw}", f"-DNF3D_BH={bh}", "-c",
+                     os.path.join(CSRC, "nf3d_shape.hip"), "-o", o])
+        objs.append(o)
+    for u in HOST_UNITS:
+        o = os.path.join(BUILD, u + ".o")
+        jobs.append([HIPCC, *HIP_FLAGS, "-x", "hip", "-c", os.path.join(CSRC, u), "-o", o])
+        objs.append(o)
+    with ThreadPoolExecutor(max_workers=min(os.cpu_count() or 8, 16, len(jobs))) as ex:
+        for out in ex.map(_run, jobs):
+            if verbose and out.strip():
+                print(out)
+    lib = os.path.join(PKG, "libnf3d.so")
+    tmp = lib + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, lib)
+    return lib
+
+
+def build_synth() -> str:
+    lib = os.path.join(PKG, "libnf3d_synth.so")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp",
+          os.path.join(CSRC, "synth.cpp"), "-o", lib + ".tmp"])
+    os.replace(lib + ".tmp", lib)
+    return lib
+
+
+def main(argv=None) -> int:
+    verbose = "-v" in (argv or sys.argv[1:])
+    build_synth()
+    lib = build_product(verbose)
+    print(f"built {lib}")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,727 @@
+// Per-plane engine of the MI355X denoiser: the C ABI's nf3d_engine_*.
+//
+// Host side of what FFT3DEngine does (src/fft3d_engine.h:87-741): parameter
+// scaling and validation, block geometry, the window / sharpen / dehalo /
+// pattern tables and the gridsample spectrum (built once, on the host, in the
+// reference's float arithmetic -- the DFT of gridsample in double), and the
+// per-frame orchestration of the gfx950 kernels:
+//
+//   bt >= 2 : spectra of the frames in [n - bt/2, n + (bt-1)/2] are kept in an
+//             HBM ring keyed by frame number (replacing the LRU frame cache of
+//             src/cache.hpp); a frame missing from the ring is transformed
+//             once (MODE_FWD), then MODE_W3D filters frame n from the ring.
+//   bt == 1, edge frames of bt >= 2 : MODE_W2D, all on chip.
+//   bt == 0 : MODE_KALMAN with the recursion state resident in HBM.
+//   bt == -1: MODE_SHARPEN.
+//   then k_ola writes the destination plane.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/nf3d.h"
+#include "nf3d_internal.h"
+
+using nf3d::BlockJob;
+using nf3d::OlaJob;
+using nf3d::PatternJob;
+
+namespace {
+
+struct Failure {
+  std::string msg;
+};
+
+[[noreturn]] void fail(const std::string& m) { throw Failure{m}; }
+
+void check(hipError_t e, const char* what) {
+  if (e != hipSuccess) fail(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+}
+
+void copy_err(char* err, size_t len, const std::string& m) {
+  if (err && len) {
+    size_t n = std::min(len - 1, m.size());
+    memcpy(err, m.data(), n);
+    err[n] = 0;
+  }
+}
+
+template <typename T>
+T* dev_alloc(size_t count) {
+  void* p = nullptr;
+  check(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)), "hipMalloc");
+  return static_cast<T*>(p);
+}
+
+}  // namespace
+
+struct nf3d_engine {
+  nf3d_params ep{};  // scaled copy, as FFT3DEngine keeps it
+  nf3d_format fmt{};
+  int plane = 0, chroma = 0, frames = 0, device = 0;
+  int framewidth = 0, frameheight = 0, lc = 0, tc = 0, rc = 0, bc = 0, Wc = 0, Hc = 0;
+  int nox = 0, noy = 0, nb = 0, outwidth = 0, mirw = 0, mirh = 0, pix = 1;
+  float norm = 0, sigma2d = 0, smin_n = 0, smax_n = 0, ht2n = 0;
+  std::vector<float> wanx, wany, wsynxl, wsynxr, wsynyl, wsynyr, wsharpen, wdehalo, pwin, pattern2d, pattern3d;
+  std::vector<float2> grid;
+  bool pattern_set = false, pattern3d_init = false;
+
+  hipStream_t stream = nullptr;
+  float* d_tab = nullptr;  // every float table, one allocation
+  float *d_wanx, *d_wany, *d_wsynxl, *d_wsynxr, *d_wsynyl, *d_wsynyr, *d_wsharpen, *d_wdehalo, *d_pwin,
+      *d_pattern2d, *d_pattern3d;
+  float2* d_grid = nullptr;
+  size_t spec_elems = 0;  // nb * bh * outwidth
+  float2* d_ring = nullptr;
+  int ring_slots = 0;
+  std::vector<int> ring_frame;
+  std::vector<uint64_t> ring_tick;
+  uint64_t tick = 0;
+  float2 *d_kal_last = nullptr, *d_kal_covar = nullptr, *d_kal_cproc = nullptr;
+  float* d_blocks = nullptr;
+  float2* d_spec_tmp = nullptr;
+  float* d_psums = nullptr;
+  int* d_choice = nullptr;
+  // staging for host-fed planes
+  std::vector<uint8_t*> d_stage;
+  std::vector<int> stage_frame;
+  size_t stage_pitch = 0;
+  int stage_next = 0;
+  uint8_t* d_dst = nullptr;
+  std::mutex mu;
+
+  ~nf3d_engine() {
+    hipFree(d_tab);
+    hipFree(d_grid);
+    hipFree(d_ring);
+    hipFree(d_kal_last);
+    hipFree(d_kal_covar);
+    hipFree(d_kal_cproc);
+    hipFree(d_blocks);
+    hipFree(d_spec_tmp);
+    hipFree(d_psums);
+    hipFree(d_choice);
+    for (auto* p : d_stage) hipFree(p);
+    hipFree(d_dst);
+    if (stream) hipStreamDestroy(stream);
+  }
+
+  // ------------------------------------------------------------- tables
+  void build_windows() {
+    // src/fft3d_engine.h:204-283, float arithmetic and glibc cosf
+    const int ow = ep.ow, oh = ep.oh;
+    const float pi = 3.1415926535897932384626433832795f;
+    std::vector<float> axl(ow), axr(ow), ayl(oh), ayr(oh);
+    wsynxl.assign(ow, 0);
+    wsynxr.assign(ow, 0);
+    wsynyl.assign(oh, 0);
+    wsynyr.assign(oh, 0);
+    auto cl = [&](int i, int o) { return cosf(pi * ((float)(i - o) + 0.5f) / (float)(o * 2)); };
+    auto cr = [&](int i, int o) { return cosf(pi * ((float)i + 0.5f) / (float)(o * 2)); };
+    if (ep.wintype == 0) {
+      for (int i = 0; i < ow; i++) { axl[i] = cl(i, ow); axr[i] = cr(i, ow); wsynxl[i] = axl[i]; wsynxr[i] = axr[i]; }
+      for (int i = 0; i < oh; i++) { ayl[i] = cl(i, oh); ayr[i] = cr(i, oh); wsynyl[i] = ayl[i]; wsynyr[i] = ayr[i]; }
+    } else if (ep.wintype == 1) {
+      for (int i = 0; i < ow; i++) {
+        axl[i] = (float)std::sqrt((double)cl(i, ow));
+        axr[i] = (float)std::sqrt((double)cr(i, ow));
+        wsynxl[i] = axl[i] * axl[i] * axl[i];
+        wsynxr[i] = axr[i] * axr[i] * axr[i];
+      }
+      for (int i = 0; i < oh; i++) {
+        ayl[i] = (float)std::sqrt((double)cl(i, oh));
+        ayr[i] = (float)std::sqrt((double)cr(i, oh));
+        wsynyl[i] = ayl[i] * ayl[i] * ayl[i];
+        wsynyr[i] = ayr[i] * ayr[i] * ayr[i];
+      }
+    } else {
+      for (int i = 0; i < ow; i++) {
+        axl[i] = axr[i] = 1.0f;
+        float a = cl(i, ow), b = cr(i, ow);
+        wsynxl[i] = a * a;
+        wsynxr[i] = b * b;
+      }
+      for (int i = 0; i < oh; i++) {
+        ayl[i] = ayr[i] = 1.0f;
+        float a = cl(i, oh), b = cr(i, oh);
+        wsynyl[i] = a * a;
+        wsynyr[i] = b * b;
+      }
+    }
+    wanx.assign(ep.bw, 1.0f);
+    wany.assign(ep.bh, 1.0f);
+    for (int j = 0; j < ow; j++) {
+      wanx[j] = axl[j];
+      wanx[ep.bw - ow + j] = axr[j];
+    }
+    for (int i = 0; i < oh; i++) {
+      wany[i] = ayl[i];
+      wany[ep.bh - oh + i] = ayr[i];
+    }
+  }
+
+  void build_tables() {
+    // src/fft3d_engine.h:285-361
+    const int bw = ep.bw, bh = ep.bh, ow = outwidth;
+    wsharpen.assign((size_t)bh * ow, 0);
+    wdehalo.assign((size_t)bh * ow, 0);
+    pwin.assign((size_t)bh * ow, 0);
+    for (int j = 0; j < bh; j++) {
+      int dj = j >= bh / 2 ? bh - j : j;
+      float d2v = (float)(dj * dj) * (ep.svr * ep.svr) / (float)((bh / 2) * (bh / 2));
+      for (int i = 0; i < ow; i++) {
+        float d2 = d2v + (float)(i * i) / (float)((bw / 2) * (bw / 2));
+        wsharpen[j * ow + i] = (float)(1 - std::exp((double)(-d2 / (2 * ep.scutoff * ep.scutoff))));
+      }
+    }
+    float wmax = 0;
+    for (int j = 0; j < bh; j++) {
+      int dj = j >= bh / 2 ? bh - j : j;
+      float d2v = (float)(dj * dj) * (ep.svr * ep.svr) / (float)((bh / 2) * (bh / 2));
+      for (int i = 0; i < ow; i++) {
+        float d2 = d2v + (float)(i * i) / (float)((bw / 2) * (bw / 2));
+        float v = (float)(std::exp((double)(-0.7f * d2 * ep.hr * ep.hr)) - std::exp((double)(-d2 * ep.hr * ep.hr)));
+        wdehalo[j * ow + i] = v;
+        if (v > wmax) wmax = v;
+      }
+    }
+    for (auto& v : wdehalo) v /= wmax;
+    for (int j = 0; j < bh; j++) {
+      int dj = std::min(j, bh - j);
+      float fh = (float)dj * 2.0f / (float)bh;
+      float fh2 = fh * fh;
+      for (int i = 0; i < ow; i++) {
+        float fw = (float)i * 2.0f / (float)bw;
+        pwin[j * ow + i] = (fh2 + fw * fw) / (fh2 + fw * fw + ep.pcutoff * ep.pcutoff);
+      }
+    }
+    pattern2d.assign((size_t)bh * ow, 0);
+    pattern3d.assign((size_t)bh * ow, 0);
+    if ((ep.sigma2 != ep.sigma || ep.sigma3 != ep.sigma || ep.sigma4 != ep.sigma) && ep.pfactor == 0) {
+      // SigmasToPattern, src/helper.cpp:14-43
+      float ft2 = (float)std::sqrt((double)0.5f) / 2;
+      float ft3 = (float)std::sqrt((double)0.5f) / 4;
+      for (int h = 0; h < bh; h++)
+        for (int w = 0; w < ow; w++) {
+          float fy = ((float)bh - 2.0f * (float)std::abs(h - bh / 2)) / (float)bh;
+          float fx = ((float)w * 1.0f) / (float)ow;
+          float f = (float)std::sqrt((double)((fx * fx + fy * fy) * 0.5f));
+          float s;
+          if (f < ft3) s = ep.sigma4 + (ep.sigma3 - ep.sigma4) * f / ft3;
+          else if (f < ft2) s = ep.sigma3 + (ep.sigma2 - ep.sigma3) * (f - ft3) / (ft2 - ft3);
+          else s = ep.sigma + (ep.sigma2 - ep.sigma) * (1 - f) / (1 - ft2);
+          pattern2d[h * ow + w] = s * s / norm;
+        }
+      pattern_set = true;
+      ep.pfactor = 1;
+    }
+  }
+
+  // Spectrum of block (0,0) of a constant maximum-value cover without chroma
+  // centring (src/fft3d_engine.h:384-404), with the gather's float products
+  // (corner block: top rows (wx*wy)*p or (wy*p)*wx, band rows (wx*p)*wy) and a
+  // double-precision DFT rounded once to float.
+  void build_grid() {
+    const int bw = ep.bw, bh = ep.bh, ow = ep.ow, oh = ep.oh;
+    float maxv = fmt.is_float ? 1.0f : (float)((1 << fmt.bits_per_sample) - 1);
+    std::vector<double> blk((size_t)bw * bh);
+    for (int i = 0; i < bh; i++)
+      for (int j = 0; j < bw; j++) {
+        bool xw = j < ow || j >= bw - ow, yw = i < oh || i >= bh - oh;
+        float wx = wanx[j], wy = wany[i], p = maxv, v;
+        if (xw && yw) {
+          bool yedge = i < oh;   // block row 0 (noy >= 2)
+          bool xedge = j < ow;   // block column 0 (nox >= 2)
+          if (yedge) v = xedge ? (wx * wy) * p : (wy * p) * wx;
+          else v = (wx * p) * wy;
+        } else if (xw) v = wx * p;
+        else if (yw) v = wy * p;
+        else v = p;
+        blk[(size_t)i * bw + j] = v;
+      }
+    grid.assign((size_t)bh * outwidth, make_float2(0, 0));
+    const double tp = 6.283185307179586476925286766559;
+    for (int k1 = 0; k1 < bh; k1++)
+      for (int k2 = 0; k2 < outwidth; k2++) {
+        double sr = 0, si = 0;
+        for (int i = 0; i < bh; i++)
+          for (int j = 0; j < bw; j++) {
+            double a = -tp * ((double)((long)k1 * i % bh) / bh + (double)((long)k2 * j % bw) / bw);
+            sr += blk[(size_t)i * bw + j] * std::cos(a);
+            si += blk[(size_t)i * bw + j] * std::sin(a);
+          }
+        grid[(size_t)k1 * outwidth + k2] = make_float2((float)sr, (float)si);
+      }
+  }
+
+  void upload_tables() {
+    const size_t sp = (size_t)ep.bh * outwidth;
+    size_t total = ep.bw + ep.bh + 2 * ep.ow + 2 * ep.oh + 5 * sp + 16;
+    d_tab = dev_alloc<float>(total);
+    std::vector<float> h(total, 0.0f);
+    size_t o = 0;
+    auto put = [&](const std::vector<float>& v, float** dptr) {
+      *dptr = d_tab + o;
+      std::copy(v.begin(), v.end(), h.begin() + o);
+      o += v.size();
+    };
+    put(wanx, &d_wanx);
+    put(wany, &d_wany);
+    put(wsynxl, &d_wsynxl);
+    put(wsynxr, &d_wsynxr);
+    put(wsynyl, &d_wsynyl);
+    put(wsynyr, &d_wsynyr);
+    put(wsharpen, &d_wsharpen);
+    put(wdehalo, &d_wdehalo);
+    put(pwin, &d_pwin);
+    put(pattern2d, &d_pattern2d);
+    put(pattern3d, &d_pattern3d);
+    check(hipMemcpy(d_tab, h.data(), total * sizeof(float), hipMemcpyHostToDevice), "tables");
+    d_grid = dev_alloc<float2>(sp);
+    check(hipMemcpy(d_grid, grid.data(), sp * sizeof(float2), hipMemcpyHostToDevice), "grid");
+  }
+
+  // ------------------------------------------------------------ creation
+  void init(const nf3d_params& p, const nf3d_format& f, int width, int height, int nframes, int pl) {
+    ep = p;
+    fmt = f;
+    plane = pl;
+    frames = nframes;
+    chroma = f.color_family == 1 && pl != 0;
+    framewidth = chroma ? width >> f.ssw : width;
+    frameheight = chroma ? height >> f.ssh : height;
+    pix = f.is_float ? 4 : f.bits_per_sample <= 8 ? 1 : 2;
+    // src/fft3d_engine.h:91-136
+    float factor = pix == 1 ? 1.0f : pix == 2 ? (float)(1 << (f.bits_per_sample - 8)) : 1 / 255.0f;
+    ep.sigma *= factor;
+    ep.sigma2 *= factor;
+    ep.sigma3 *= factor;
+    ep.sigma4 *= factor;
+    ep.smin *= factor;
+    ep.smax *= factor;
+    if (ep.ow * 2 > ep.bw) fail("ow must be less than bw / 2");
+    if (ep.oh * 2 > ep.bh) fail("oh must be less than bh / 2");
+    if (ep.ow < 0) ep.ow = ep.bw / 3;
+    if (ep.oh < 0) ep.oh = ep.bh / 3;
+    if (ep.bt < -1 || ep.bt > 5) fail("bt must be -1 (Sharpen), 0 (Kalman), 1..5 (Wiener)");
+    int xs = chroma ? f.ssw : 0, ys = chroma ? f.ssh : 0;
+    const int bw = ep.bw, bh = ep.bh, ow = ep.ow, oh = ep.oh;
+    if (bw <= 0 || bh <= 0 || bw - ow <= 0 || bh - oh <= 0) fail("block size must be positive");
+    nox = (((width - ep.l - ep.r) >> xs) - ow + (bw - ow - 1)) / (bw - ow) + 2;
+    noy = (((height - ep.t - ep.b) >> ys) - oh + (bh - oh - 1)) / (bh - oh) + 2;
+    mirw = bw - ow;
+    mirh = bh - oh;
+    if (ep.beta < 1) fail("beta must be not less 1.0");
+    nb = nox * noy;
+    outwidth = bw / 2 + 1;
+    lc = chroma ? ep.l >> f.ssw : ep.l;
+    rc = chroma ? ep.r >> f.ssw : ep.r;
+    tc = chroma ? ep.t >> f.ssh : ep.t;
+    bc = chroma ? ep.b >> f.ssh : ep.b;
+    Wc = framewidth - lc - rc;
+    Hc = frameheight - tc - bc;
+    // Cover padding is one reflect-101 of the plane (src/buffer.cpp:19-26,
+    // 65-79); planes smaller than the padding would make the reference read
+    // unfilled cover memory, so they are refused here.
+    int coverw = nox * (bw - ow) + ow, coverh = noy * (bh - oh) + oh;
+    if (Wc < 2 || Hc < 2 || mirw > Wc - 1 || mirh > Hc - 1 || coverw - mirw - Wc > Wc - 1 ||
+        coverh - mirh - Hc > Hc - 1)
+      fail("plane is too small for the block size");
+    // every copied pixel must lie inside the block grid (an odd crop of a
+    // subsampled plane can make the copied width exceed the geometry width)
+    if ((mirw + Wc - 1) / (bw - ow) > nox - 1 || (mirh + Hc - 1) / (bh - oh) > noy - 1)
+      fail("crop leaves the plane outside the block grid");
+    if (ep.px < 0 || ep.py < 0 || ep.px >= nox || ep.py >= noy) fail("px/py outside the block grid");
+    if (ep.interlaced && (Hc & 1)) fail("interlaced requires an even plane height");
+    if (ep.pshow) fail("pshow is not supported by the GPU engine");
+    if (!nf3d::block_shape_supported(bw, bh)) fail("block size bw x bh not compiled for the GPU engine");
+    build_windows();
+    norm = 1.0f / (float)(bw * bh);
+    sigma2d = ep.sigma * ep.sigma / norm;
+    smin_n = ep.smin * ep.smin / norm;
+    smax_n = ep.smax * ep.smax / norm;
+    ht2n = ep.ht * ep.ht / norm;
+    build_tables();
+    build_grid();
+
+    check(hipGetDevice(&device), "hipGetDevice");
+    check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+    upload_tables();
+    spec_elems = (size_t)nb * bh * outwidth;
+    d_blocks = dev_alloc<float>((size_t)nb * bh * bw);
+    if (ep.bt >= 2) {
+      ring_slots = ep.bt + 2;
+      d_ring = dev_alloc<float2>(spec_elems * ring_slots);
+      ring_frame.assign(ring_slots, -1);
+      ring_tick.assign(ring_slots, 0);
+    }
+    if (ep.bt == 0) {
+      d_kal_last = dev_alloc<float2>(spec_elems);
+      d_kal_covar = dev_alloc<float2>(spec_elems);
+      d_kal_cproc = dev_alloc<float2>(spec_elems);
+      std::vector<float2> h(spec_elems, make_float2(0, 0));
+      check(hipMemcpy(d_kal_last, h.data(), spec_elems * sizeof(float2), hipMemcpyHostToDevice), "kalman");
+      std::fill(h.begin(), h.end(), make_float2(sigma2d, sigma2d));
+      check(hipMemcpy(d_kal_covar, h.data(), spec_elems * sizeof(float2), hipMemcpyHostToDevice), "kalman");
+      check(hipMemcpy(d_kal_cproc, h.data(), spec_elems * sizeof(float2), hipMemcpyHostToDevice), "kalman");
+    }
+  }
+
+  // ------------------------------------------------------------ per frame
+  BlockJob base_job() const {
+    BlockJob J{};
+    J.nox = nox;
+    J.noy = noy;
+    J.nb = nb;
+    J.ow = ep.ow;
+    J.oh = ep.oh;
+    J.mirw = mirw;
+    J.mirh = mirh;
+    J.Wc = Wc;
+    J.Hc = Hc;
+    J.interlaced = ep.interlaced;
+    J.lc = lc;
+    J.tc = tc;
+    J.pix = pix;
+    J.base = (chroma && !fmt.is_float) ? (float)(1 << (fmt.bits_per_sample - 1)) : 0.0f;
+    J.wanx = d_wanx;
+    J.wany = d_wany;
+    J.grid = d_grid;
+    J.wsharpen = d_wsharpen;
+    J.wdehalo = d_wdehalo;
+    J.pattern2d = d_pattern2d;
+    J.pattern3d = d_pattern3d;
+    J.sigma2d = sigma2d;
+    J.lowlimit = (ep.beta - 1) / ep.beta;
+    J.degrid = ep.degrid;
+    J.sharpen = ep.sharpen;
+    J.smin_n = smin_n;
+    J.smax_n = smax_n;
+    J.dehalo = ep.dehalo;
+    J.ht2n = ht2n;
+    J.kratio2 = ep.kratio * ep.kratio;
+    J.use_pattern = ep.pfactor != 0;
+    J.use_degrid = ep.degrid != 0;
+    J.kal_last = d_kal_last;
+    J.kal_covar = d_kal_covar;
+    J.kal_cproc = d_kal_cproc;
+    J.blocks = d_blocks;
+    return J;
+  }
+
+  struct Src {
+    const uint8_t* p;
+    long long pitch;
+  };
+
+  struct Call {
+    const nf3d_frame_ref* frames;
+    int nframes;
+    bool device;
+    hipStream_t s;
+  };
+
+  const nf3d_frame_ref* find(const Call& c, int k) const {
+    for (int i = 0; i < c.nframes; i++)
+      if (c.frames[i].n == k) return &c.frames[i];
+    return nullptr;
+  }
+
+  Src source(const Call& c, int k) {
+    const nf3d_frame_ref* fr = find(c, k);
+    if (!fr || !fr->data) fail("frame " + std::to_string(k) + " was not provided");
+    if (c.device) return Src{static_cast<const uint8_t*>(fr->data), (long long)fr->stride};
+    // host plane -> staging slot
+    for (size_t i = 0; i < stage_frame.size(); i++)
+      if (stage_frame[i] == k) return Src{d_stage[i], (long long)stage_pitch};
+    if (d_stage.empty()) {
+      stage_pitch = ((size_t)framewidth * pix + 255) / 256 * 256;
+      for (int i = 0; i < 7; i++) d_stage.push_back(dev_alloc<uint8_t>(stage_pitch * frameheight));
+      stage_frame.assign(d_stage.size(), -1);
+    }
+    int slot = stage_next;
+    stage_next = (stage_next + 1) % (int)d_stage.size();
+    stage_frame[slot] = k;
+    check(hipMemcpy2DAsync(d_stage[slot], stage_pitch, fr->data, fr->stride, (size_t)framewidth * pix,
+                           frameheight, hipMemcpyHostToDevice, c.s),
+          "upload");
+    return Src{d_stage[slot], (long long)stage_pitch};
+  }
+
+  void launch(int mode, int bt, const BlockJob& J, hipStream_t s) {
+    check(nf3d::launch_block(mode, bt, ep.bw, ep.bh, J, s), "block kernel");
+  }
+
+  int ring_lookup(int k) {
+    for (int i = 0; i < ring_slots; i++)
+      if (ring_frame[i] == k) {
+        ring_tick[i] = ++tick;
+        return i;
+      }
+    return -1;
+  }
+
+  int ring_claim(int k, int lo, int hi) {
+    int best = -1;
+    for (int i = 0; i < ring_slots; i++) {
+      if (ring_frame[i] >= lo && ring_frame[i] <= hi) continue;
+      if (best < 0 || ring_tick[i] < ring_tick[best]) best = i;
+    }
+    ring_frame[best] = k;
+    ring_tick[best] = ++tick;
+    return best;
+  }
+
+  float2* slot_ptr(int s) { return d_ring + spec_elems * (size_t)s; }
+
+  void estimate_pattern(const Call& c) {
+    // src/fft3d_engine.h:485-504 with FindPatternBlock / SetPattern
+    int pf = std::max(0, std::min(ep.pframe, frames - 1));
+    Src ps = source(c, pf);
+    if (!d_spec_tmp) {
+      d_spec_tmp = dev_alloc<float2>(spec_elems);
+      d_psums = dev_alloc<float>(nb);
+      d_choice = dev_alloc<int>(2);
+    }
+    BlockJob J = base_job();
+    J.src = ps.p;
+    J.src_pitch = ps.pitch;
+    J.spec_out = d_spec_tmp;
+    launch(nf3d::MODE_FWD, 0, J, c.s);
+    PatternJob P{};
+    P.nox = nox;
+    P.noy = noy;
+    P.bh = ep.bh;
+    P.outwidth = outwidth;
+    P.spec = d_spec_tmp;
+    P.grid = d_grid;
+    P.pwin = d_pwin;
+    P.degrid = ep.degrid;
+    P.px = ep.px;
+    P.py = ep.py;
+    P.pattern2d = d_pattern2d;
+    P.sums = d_psums;
+    P.choice = d_choice;
+    check(nf3d::launch_pattern(P, c.s), "pattern kernels");
+    int choice[2];
+    check(hipMemcpyAsync(choice, d_choice, sizeof(choice), hipMemcpyDeviceToHost, c.s), "pattern");
+    check(hipMemcpyAsync(pattern2d.data(), d_pattern2d, pattern2d.size() * sizeof(float),
+                         hipMemcpyDeviceToHost, c.s),
+          "pattern");
+    check(hipStreamSynchronize(c.s), "pattern");
+    ep.px = choice[0];
+    ep.py = choice[1];
+    pattern_set = true;
+  }
+
+  void ensure_pattern3d(int btcur, hipStream_t s) {
+    // Pattern2Dto3D with the first 3D btcur (src/fft3d_engine.h:625-630)
+    if (pattern3d_init) return;
+    for (size_t i = 0; i < pattern2d.size(); i++) pattern3d[i] = pattern2d[i] * (float)btcur;
+    check(hipMemcpyAsync(d_pattern3d, pattern3d.data(), pattern3d.size() * sizeof(float),
+                         hipMemcpyHostToDevice, s),
+          "pattern3d");
+    check(hipStreamSynchronize(s), "pattern3d");
+    pattern3d_init = true;
+  }
+
+  void run(int n, const Call& c, uint8_t* dst, long long dst_pitch) {
+    check(hipSetDevice(device), "hipSetDevice");
+    if (ep.pfactor != 0 && !pattern_set) estimate_pattern(c);
+    int btcur = ep.bt;
+    if ((ep.bt / 2 > n) || (ep.bt - 1) / 2 > (frames - 1 - n)) btcur = 1;
+    BlockJob J = base_job();
+    J.sigma2 = (float)btcur * ep.sigma * ep.sigma / norm;
+    if (ep.bt == 0 && n == 0) {
+      // the first frame passes through unfiltered (src/fft3d_engine.h:681-687)
+      Src s0 = source(c, 0);
+      check(hipMemcpy2DAsync(dst + (long long)tc * dst_pitch + (long long)lc * pix, dst_pitch,
+                             s0.p + (long long)tc * s0.pitch + (long long)lc * pix, s0.pitch,
+                             (size_t)Wc * pix, Hc, hipMemcpyDeviceToDevice, c.s),
+            "copy");
+      return;
+    }
+    if (btcur > 0) {
+      if (btcur == 1) {
+        Src s0 = source(c, n);
+        J.src = s0.p;
+        J.src_pitch = s0.pitch;
+        launch(nf3d::MODE_W2D, 0, J, c.s);
+      } else {
+        if (J.use_pattern) ensure_pattern3d(btcur, c.s);
+        int from = -btcur / 2, to = (btcur - 1) / 2;
+        for (int i = from; i <= to; i++) {
+          int k = n + i;
+          int slot = ring_lookup(k);
+          if (slot < 0) {
+            slot = ring_claim(k, n + from, n + to);
+            Src sk = source(c, k);
+            BlockJob F = base_job();
+            F.src = sk.p;
+            F.src_pitch = sk.pitch;
+            F.spec_out = slot_ptr(slot);
+            launch(nf3d::MODE_FWD, 0, F, c.s);
+          }
+          J.spec_in[i - from] = slot_ptr(slot);
+          if (i == 0) J.spec_cur = slot_ptr(slot);
+        }
+        launch(nf3d::MODE_W3D, btcur, J, c.s);
+      }
+    } else {
+      Src s0 = source(c, n);
+      J.src = s0.p;
+      J.src_pitch = s0.pitch;
+      launch(ep.bt == 0 ? nf3d::MODE_KALMAN : nf3d::MODE_SHARPEN, 0, J, c.s);
+    }
+    OlaJob O{};
+    O.nox = nox;
+    O.noy = noy;
+    O.bw = ep.bw;
+    O.bh = ep.bh;
+    O.ow = ep.ow;
+    O.oh = ep.oh;
+    O.mirw = mirw;
+    O.mirh = mirh;
+    O.Wc = Wc;
+    O.Hc = Hc;
+    O.interlaced = ep.interlaced;
+    O.blocks = d_blocks;
+    O.wsynxl = d_wsynxl;
+    O.wsynxr = d_wsynxr;
+    O.wsynyl = d_wsynyl;
+    O.wsynyr = d_wsynyr;
+    O.norm = norm;
+    O.dst = dst;
+    O.dst_pitch = dst_pitch;
+    O.lc = lc;
+    O.tc = tc;
+    O.pix = pix;
+    O.base = (chroma && !fmt.is_float) ? (1 << (fmt.bits_per_sample - 1)) : 0;
+    O.maxval = fmt.is_float ? 1 : (1 << fmt.bits_per_sample) - 1;
+    check(nf3d::launch_ola(O, c.s), "overlap kernel");
+  }
+
+  void get_frame_host(int n, const nf3d_frame_ref* frames_, int nframes, void* dst, int dst_stride) {
+    Call c{frames_, nframes, false, stream};
+    std::fill(stage_frame.begin(), stage_frame.end(), -1);
+    size_t dpitch = ((size_t)framewidth * pix + 255) / 256 * 256;
+    if (!d_dst) d_dst = dev_alloc<uint8_t>(dpitch * frameheight);
+    run(n, c, d_dst, (long long)dpitch);
+    check(hipMemcpy2DAsync(static_cast<uint8_t*>(dst) + (size_t)tc * dst_stride + (size_t)lc * pix, dst_stride,
+                           d_dst + (size_t)tc * dpitch + (size_t)lc * pix, dpitch, (size_t)Wc * pix, Hc,
+                           hipMemcpyDeviceToHost, stream),
+          "download");
+    check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+  }
+};
+
+// ---------------------------------------------------------------- C ABI
+extern "C" {
+
+void nf3d_params_defaults(nf3d_params* p, float sigma) {
+  // src/fft3d.hpp:77-106
+  *p = nf3d_params{};
+  p->sigma = sigma;
+  p->beta = 1.0f;
+  p->bw = 32;
+  p->bh = 32;
+  p->bt = 3;
+  p->ow = -1;
+  p->oh = -1;
+  p->kratio = 2.0f;
+  p->sharpen = 0.0f;
+  p->scutoff = 0.3f;
+  p->svr = 1.0f;
+  p->smin = 4.0f;
+  p->smax = 20.0f;
+  p->measure = 1;
+  p->interlaced = 0;
+  p->wintype = 0;
+  p->pframe = 0;
+  p->px = 0;
+  p->py = 0;
+  p->pshow = 0;
+  p->pcutoff = 0.1f;
+  p->pfactor = 0.0f;
+  p->sigma2 = sigma;
+  p->sigma3 = sigma;
+  p->sigma4 = sigma;
+  p->degrid = 1.0f;
+  p->dehalo = 0.0f;
+  p->hr = 2.0f;
+  p->ht = 50.0f;
+  p->l = p->t = p->r = p->b = 0;
+  p->opt = 0;
+}
+
+int nf3d_engine_create(const nf3d_params* p, const nf3d_format* fmt, int width, int height, int frames,
+                       int plane, nf3d_engine** out, char* err, size_t errlen) {
+  if (!p || !fmt || !out) {
+    copy_err(err, errlen, "null argument");
+    return -1;
+  }
+  auto* e = new nf3d_engine();
+  try {
+    e->init(*p, *fmt, width, height, frames, plane);
+  } catch (const Failure& f) {
+    copy_err(err, errlen, f.msg);
+    delete e;
+    return -1;
+  }
+  *out = e;
+  return 0;
+}
+
+int nf3d_engine_get_frame(nf3d_engine* e, int n, const nf3d_frame_ref* frames, int nframes, void* dst,
+                          int dst_stride, char* err, size_t errlen) {
+  std::lock_guard<std::mutex> lock(e->mu);
+  try {
+    e->get_frame_host(n, frames, nframes, dst, dst_stride);
+  } catch (const Failure& f) {
+    copy_err(err, errlen, f.msg);
+    return -1;
+  }
+  return 0;
+}
+
+int nf3d_engine_get_frame_device(nf3d_engine* e, int n, const nf3d_frame_ref* frames, int nframes, void* dst,
+                                 int dst_stride, void* stream, char* err, size_t errlen) {
+  std::lock_guard<std::mutex> lock(e->mu);
+  try {
+    nf3d_engine::Call c{frames, nframes, true, stream ? static_cast<hipStream_t>(stream) : e->stream};
+    e->run(n, c, static_cast<uint8_t*>(dst), dst_stride);
+  } catch (const Failure& f) {
+    copy_err(err, errlen, f.msg);
+    return -1;
+  }
+  return 0;
+}
+
+int nf3d_engine_cache_refresh(nf3d_engine* e, int n) {
+  std::lock_guard<std::mutex> lock(e->mu);
+  if (e->ring_slots == 0) return 0;
+  return e->ring_lookup(n) >= 0 ? 1 : 0;
+}
+
+int nf3d_engine_geometry(const nf3d_engine* e, int* o) {
+  o[0] = e->nox;
+  o[1] = e->noy;
+  o[2] = e->outwidth;
+  o[3] = e->mirw;
+  o[4] = e->mirh;
+  o[5] = e->ep.ow;
+  o[6] = e->ep.oh;
+  return 0;
+}
+
+int nf3d_engine_sync(nf3d_engine* e) { return hipStreamSynchronize(e->stream) == hipSuccess ? 0 : -1; }
+
+void nf3d_engine_destroy(nf3d_engine* e) { delete e; }
+
+}  // extern "C"
