@@ -86,12 +86,27 @@ int nf3d_engine_get_frame(nf3d_engine* e, int n, const nf3d_frame_ref* frames, i
 int nf3d_engine_get_frame_device(nf3d_engine* e, int n, const nf3d_frame_ref* frames, int nframes,
                                  void* dst, int dst_stride, void* stream, char* err, size_t errlen);
 
+/* Throughput form of nf3d_engine_get_frame_device for a device-resident clip
+ * (no reference counterpart; the reference is driven one GetFrame at a time):
+ * filters frames [first, first+count) in order.  src[k] / dst[k] are this
+ * plane of frame k of the clip (k < frames), all with the given strides.  The
+ * work is enqueued on `stream`; the call returns without waiting. */
+int nf3d_engine_run_device(nf3d_engine* e, int first, int count, const void* const* src, int src_stride,
+                           void* const* dst, int dst_stride, void* stream, char* err, size_t errlen);
+
 /* Replaces FFT3DEngine::CacheRefresh (src/fft3d_engine.h:736-740): 1 if the
  * spectrum of frame n is held in the HBM spectra ring (and marks it recent). */
 int nf3d_engine_cache_refresh(nf3d_engine* e, int n);
 
 /* Block geometry of the engine: nox, noy, outwidth, mirw, mirh, ow, oh. */
 int nf3d_engine_geometry(const nf3d_engine* e, int* out7);
+
+/* Diagnostics (no reference counterpart): when enabled, the main spectral
+ * kernel of every frame is bracketed by HIP events on its stream;
+ * nf3d_engine_profile_read waits for them and returns the summed kernel time
+ * (ms) and the number of timed launches since the last read. */
+int nf3d_engine_profile(nf3d_engine* e, int enable);
+int nf3d_engine_profile_read(nf3d_engine* e, double* total_ms, int* launches);
 
 /* Wait for the engine's queued work (the host-fed calls already do). */
 int nf3d_engine_sync(nf3d_engine* e);

@@ -8,12 +8,17 @@
 //
 //   bt >= 2 : spectra of the frames in [n - bt/2, n + (bt-1)/2] are kept in an
 //             HBM ring keyed by frame number (replacing the LRU frame cache of
-//             src/cache.hpp); a frame missing from the ring is transformed
-//             once (MODE_FWD), then MODE_W3D filters frame n from the ring.
+//             src/cache.hpp).  In order, the newest frame of the window is the
+//             only one not yet transformed: MODE_W3D computes its spectrum on
+//             chip, stores it to the ring once, and filters frame n with the
+//             older spectra read from the ring.  Frames missing for any other
+//             reason (start of the clip, random access) are transformed first
+//             by MODE_FWD.
 //   bt == 1, edge frames of bt >= 2 : MODE_W2D, all on chip.
 //   bt == 0 : MODE_KALMAN with the recursion state resident in HBM.
 //   bt == -1: MODE_SHARPEN.
-//   then k_ola writes the destination plane.
+// Every filtering launch ends with k_seam finishing the overlap-add bands
+// shared between workgroup tiles.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,9 +33,9 @@
 #include "../../include/nf3d.h"
 #include "nf3d_internal.h"
 
-using nf3d::BlockJob;
-using nf3d::OlaJob;
 using nf3d::PatternJob;
+using nf3d::PlaneJob;
+using nf3d::TileLaunch;
 
 namespace {
 
@@ -84,7 +89,8 @@ struct nf3d_engine {
   std::vector<uint64_t> ring_tick;
   uint64_t tick = 0;
   float2 *d_kal_last = nullptr, *d_kal_covar = nullptr, *d_kal_cproc = nullptr;
-  float* d_blocks = nullptr;
+  float* d_seam = nullptr;
+  int ntx = 0;
   float2* d_spec_tmp = nullptr;
   float* d_psums = nullptr;
   int* d_choice = nullptr;
@@ -95,6 +101,10 @@ struct nf3d_engine {
   int stage_next = 0;
   uint8_t* d_dst = nullptr;
   std::mutex mu;
+  // opt-in HIP-event timing of the main spectral kernel of each frame
+  bool profiling = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
 
   ~nf3d_engine() {
     hipFree(d_tab);
@@ -103,12 +113,13 @@ struct nf3d_engine {
     hipFree(d_kal_last);
     hipFree(d_kal_covar);
     hipFree(d_kal_cproc);
-    hipFree(d_blocks);
+    hipFree(d_seam);
     hipFree(d_spec_tmp);
     hipFree(d_psums);
     hipFree(d_choice);
     for (auto* p : d_stage) hipFree(p);
     hipFree(d_dst);
+    for (auto ev : ev_pool) hipEventDestroy(ev);
     if (stream) hipStreamDestroy(stream);
   }
 
@@ -354,7 +365,9 @@ struct nf3d_engine {
     check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
     upload_tables();
     spec_elems = (size_t)nb * bh * outwidth;
-    d_blocks = dev_alloc<float>((size_t)nb * bh * bw);
+    d_seam = dev_alloc<float>((size_t)4 * Hc * Wc);
+    int bpw = nf3d::tile_blocks_per_wg(bw, bh);
+    ntx = (nox + bpw - 1) / bpw;
     if (ep.bt >= 2) {
       ring_slots = ep.bt + 2;
       d_ring = dev_alloc<float2>(spec_elems * ring_slots);
@@ -374,8 +387,8 @@ struct nf3d_engine {
   }
 
   // ------------------------------------------------------------ per frame
-  BlockJob base_job() const {
-    BlockJob J{};
+  PlaneJob base_job() const {
+    PlaneJob J{};
     J.nox = nox;
     J.noy = noy;
     J.nb = nb;
@@ -386,12 +399,24 @@ struct nf3d_engine {
     J.Wc = Wc;
     J.Hc = Hc;
     J.interlaced = ep.interlaced;
+    J.ntx = ntx;
+    J.wg_begin = 0;
+    J.seam_row_begin = 0;
     J.lc = lc;
     J.tc = tc;
     J.pix = pix;
     J.base = (chroma && !fmt.is_float) ? (float)(1 << (fmt.bits_per_sample - 1)) : 0.0f;
+    J.obase = (chroma && !fmt.is_float) ? (1 << (fmt.bits_per_sample - 1)) : 0;
+    J.maxval = fmt.is_float ? 1 : (1 << fmt.bits_per_sample) - 1;
+    J.norm = norm;
+    J.seam = d_seam;
+    J.seam_plane = (long long)Hc * Wc;
     J.wanx = d_wanx;
     J.wany = d_wany;
+    J.wsynxl = d_wsynxl;
+    J.wsynxr = d_wsynxr;
+    J.wsynyl = d_wsynyl;
+    J.wsynyr = d_wsynyr;
     J.grid = d_grid;
     J.wsharpen = d_wsharpen;
     J.wdehalo = d_wdehalo;
@@ -411,7 +436,6 @@ struct nf3d_engine {
     J.kal_last = d_kal_last;
     J.kal_covar = d_kal_covar;
     J.kal_cproc = d_kal_cproc;
-    J.blocks = d_blocks;
     return J;
   }
 
@@ -454,8 +478,40 @@ struct nf3d_engine {
     return Src{d_stage[slot], (long long)stage_pitch};
   }
 
-  void launch(int mode, int bt, const BlockJob& J, hipStream_t s) {
-    check(nf3d::launch_block(mode, bt, ep.bw, ep.bh, J, s), "block kernel");
+  void launch(int mode, int bt, const PlaneJob& J, hipStream_t s, bool timed = false, int onchip = 0) {
+    TileLaunch tl{};
+    tl.job[0] = J;
+    tl.njobs = 1;
+    tl.total_wg = noy * ntx;
+    tl.total_seam_rows = Hc;
+    tl.max_wc = Wc;
+    hipEvent_t a = nullptr, b = nullptr;
+    if (timed && profiling) {
+      while (ev_pool.size() < ev_used + 2) {
+        hipEvent_t ev;
+        check(hipEventCreate(&ev), "hipEventCreate");
+        ev_pool.push_back(ev);
+      }
+      a = ev_pool[ev_used++];
+      b = ev_pool[ev_used++];
+      check(hipEventRecord(a, s), "hipEventRecord");
+    }
+    check(nf3d::launch_tile(mode, bt, onchip, ep.bw, ep.bh, tl, s), "tile kernel");
+    if (b) check(hipEventRecord(b, s), "hipEventRecord");
+  }
+
+  // Sum of the timed launches' durations (ms) since the last read.
+  double profile_read(int* launches) {
+    double total = 0;
+    for (size_t i = 0; i + 1 < ev_used; i += 2) {
+      check(hipEventSynchronize(ev_pool[i + 1]), "hipEventSynchronize");
+      float ms = 0;
+      check(hipEventElapsedTime(&ms, ev_pool[i], ev_pool[i + 1]), "hipEventElapsedTime");
+      total += ms;
+    }
+    if (launches) *launches = (int)(ev_used / 2);
+    ev_used = 0;
+    return total;
   }
 
   int ring_lookup(int k) {
@@ -489,7 +545,7 @@ struct nf3d_engine {
       d_psums = dev_alloc<float>(nb);
       d_choice = dev_alloc<int>(2);
     }
-    BlockJob J = base_job();
+    PlaneJob J = base_job();
     J.src = ps.p;
     J.src_pitch = ps.pitch;
     J.spec_out = d_spec_tmp;
@@ -536,8 +592,10 @@ struct nf3d_engine {
     if (ep.pfactor != 0 && !pattern_set) estimate_pattern(c);
     int btcur = ep.bt;
     if ((ep.bt / 2 > n) || (ep.bt - 1) / 2 > (frames - 1 - n)) btcur = 1;
-    BlockJob J = base_job();
+    PlaneJob J = base_job();
     J.sigma2 = (float)btcur * ep.sigma * ep.sigma / norm;
+    J.dst = dst;
+    J.dst_pitch = dst_pitch;
     if (ep.bt == 0 && n == 0) {
       // the first frame passes through unfiltered (src/fft3d_engine.h:681-687)
       Src s0 = source(c, 0);
@@ -552,59 +610,44 @@ struct nf3d_engine {
         Src s0 = source(c, n);
         J.src = s0.p;
         J.src_pitch = s0.pitch;
-        launch(nf3d::MODE_W2D, 0, J, c.s);
+        // keep the spectrum for the neighbouring 3D windows (results-neutral)
+        if (ring_slots && ring_lookup(n) < 0) J.spec_out = slot_ptr(ring_claim(n, n, n));
+        launch(nf3d::MODE_W2D, 0, J, c.s, true);
       } else {
         if (J.use_pattern) ensure_pattern3d(btcur, c.s);
-        int from = -btcur / 2, to = (btcur - 1) / 2;
-        for (int i = from; i <= to; i++) {
+        const int from = -btcur / 2, to = (btcur - 1) / 2;
+        for (int i = from; i < to; i++) {  // older frames: from the ring, else transformed now
           int k = n + i;
           int slot = ring_lookup(k);
           if (slot < 0) {
             slot = ring_claim(k, n + from, n + to);
             Src sk = source(c, k);
-            BlockJob F = base_job();
+            PlaneJob F = base_job();
             F.src = sk.p;
             F.src_pitch = sk.pitch;
             F.spec_out = slot_ptr(slot);
             launch(nf3d::MODE_FWD, 0, F, c.s);
           }
           J.spec_in[i - from] = slot_ptr(slot);
-          if (i == 0) J.spec_cur = slot_ptr(slot);
         }
-        launch(nf3d::MODE_W3D, btcur, J, c.s);
+        int newest = ring_lookup(n + to);
+        int onchip = newest < 0;
+        if (onchip) {
+          Src sk = source(c, n + to);
+          J.src = sk.p;
+          J.src_pitch = sk.pitch;
+          J.spec_out = slot_ptr(ring_claim(n + to, n + from, n + to));
+        } else {
+          J.spec_in[btcur - 1] = slot_ptr(newest);
+        }
+        launch(nf3d::MODE_W3D, btcur, J, c.s, true, onchip);
       }
     } else {
       Src s0 = source(c, n);
       J.src = s0.p;
       J.src_pitch = s0.pitch;
-      launch(ep.bt == 0 ? nf3d::MODE_KALMAN : nf3d::MODE_SHARPEN, 0, J, c.s);
+      launch(ep.bt == 0 ? nf3d::MODE_KALMAN : nf3d::MODE_SHARPEN, 0, J, c.s, true);
     }
-    OlaJob O{};
-    O.nox = nox;
-    O.noy = noy;
-    O.bw = ep.bw;
-    O.bh = ep.bh;
-    O.ow = ep.ow;
-    O.oh = ep.oh;
-    O.mirw = mirw;
-    O.mirh = mirh;
-    O.Wc = Wc;
-    O.Hc = Hc;
-    O.interlaced = ep.interlaced;
-    O.blocks = d_blocks;
-    O.wsynxl = d_wsynxl;
-    O.wsynxr = d_wsynxr;
-    O.wsynyl = d_wsynyl;
-    O.wsynyr = d_wsynyr;
-    O.norm = norm;
-    O.dst = dst;
-    O.dst_pitch = dst_pitch;
-    O.lc = lc;
-    O.tc = tc;
-    O.pix = pix;
-    O.base = (chroma && !fmt.is_float) ? (1 << (fmt.bits_per_sample - 1)) : 0;
-    O.maxval = fmt.is_float ? 1 : (1 << fmt.bits_per_sample) - 1;
-    check(nf3d::launch_ola(O, c.s), "overlap kernel");
   }
 
   void get_frame_host(int n, const nf3d_frame_ref* frames_, int nframes, void* dst, int dst_stride) {
@@ -703,6 +746,32 @@ int nf3d_engine_get_frame_device(nf3d_engine* e, int n, const nf3d_frame_ref* fr
   return 0;
 }
 
+int nf3d_engine_run_device(nf3d_engine* e, int first, int count, const void* const* src, int src_stride,
+                           void* const* dst, int dst_stride, void* stream, char* err, size_t errlen) {
+  std::lock_guard<std::mutex> lock(e->mu);
+  try {
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : e->stream;
+    for (int n = first; n < first + count; n++) {
+      if (n < 0 || n >= e->frames) fail("frame " + std::to_string(n) + " outside the clip");
+      nf3d_frame_ref win[6];
+      int k = 0;
+      int lo = e->ep.bt > 1 ? std::max(n - e->ep.bt / 2, 0) : n;
+      int hi = e->ep.bt > 1 ? std::min(n + (e->ep.bt - 1) / 2, e->frames - 1) : n;
+      for (int i = lo; i <= hi; i++) win[k++] = nf3d_frame_ref{i, src[i], src_stride};
+      if (e->ep.pfactor != 0 && !e->pattern_set) {
+        int pf = std::max(0, std::min(e->ep.pframe, e->frames - 1));
+        if (pf < lo || pf > hi) win[k++] = nf3d_frame_ref{pf, src[pf], src_stride};
+      }
+      nf3d_engine::Call c{win, k, true, s};
+      e->run(n, c, static_cast<uint8_t*>(dst[n]), dst_stride);
+    }
+  } catch (const Failure& f) {
+    copy_err(err, errlen, f.msg);
+    return -1;
+  }
+  return 0;
+}
+
 int nf3d_engine_cache_refresh(nf3d_engine* e, int n) {
   std::lock_guard<std::mutex> lock(e->mu);
   if (e->ring_slots == 0) return 0;
@@ -717,6 +786,23 @@ int nf3d_engine_geometry(const nf3d_engine* e, int* o) {
   o[4] = e->mirh;
   o[5] = e->ep.ow;
   o[6] = e->ep.oh;
+  return 0;
+}
+
+int nf3d_engine_profile(nf3d_engine* e, int enable) {
+  std::lock_guard<std::mutex> lock(e->mu);
+  e->profiling = enable != 0;
+  e->ev_used = 0;
+  return 0;
+}
+
+int nf3d_engine_profile_read(nf3d_engine* e, double* total_ms, int* launches) {
+  std::lock_guard<std::mutex> lock(e->mu);
+  try {
+    *total_ms = e->profile_read(launches);
+  } catch (const Failure&) {
+    return -1;
+  }
   return 0;
 }
 

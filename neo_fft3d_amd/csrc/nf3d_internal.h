@@ -10,28 +10,40 @@ namespace nf3d {
 enum Mode : int {
   MODE_FWD = 0,      // gather + window + 2D r2c -> spectrum store (ring fill)
   MODE_W2D = 1,      // gather + r2c + Wiener2D (+sharpen/dehalo) + c2r
-  MODE_W3D = 2,      // bt spectra (ring) + Wiener3D + sharpen + c2r
+  MODE_W3D = 2,      // bt spectra (ring, newest optionally on chip) + Wiener3D + sharpen + c2r
   MODE_KALMAN = 3,   // gather + r2c + Kalman (state in HBM) + sharpen + c2r
   MODE_SHARPEN = 4,  // gather + r2c + sharpen/dehalo + c2r
 };
 
 // Everything one launch needs about one plane of one frame.
-struct BlockJob {
+struct PlaneJob {
   // block geometry (src/fft3d_engine.h:112-136)
   int nox, noy, nb;
   int ow, oh;
   int mirw, mirh;
   int Wc, Hc;         // copied region (cropped plane size)
   int interlaced;
+  int ntx;            // tiles per block row: ceil(nox / blocks per workgroup)
+  int wg_begin;       // first workgroup of this plane in the launch
+  int seam_row_begin; // first k_seam grid row of this plane
   // source pixels
   const uint8_t* src; // device plane base (row 0 of the uncropped plane)
   long long src_pitch;  // bytes
   int lc, tc;         // crop offsets in pixels
   int pix;            // bytes per sample: 1, 2 or 4 (float)
   float base;         // chroma centring subtracted in the gather (0 for luma and float)
+  // destination
+  uint8_t* dst;
+  long long dst_pitch;
+  int obase;          // planeBase added back on output (integer chroma)
+  int maxval;         // 2^bits - 1 (integer formats)
+  float norm;         // 1 / (bw*bh)
+  float* seam;        // [4][Hc][Wc] partial sums of bands shared by tiles
+  long long seam_plane;  // Hc * Wc
   // tables (device)
   const float* wanx;  // [bw] analysis window along x (1 in the middle)
   const float* wany;  // [bh]
+  const float* wsynxl; const float* wsynxr; const float* wsynyl; const float* wsynyr;
   const float2* grid; // [bh*outwidth] gridsample
   const float* wsharpen;
   const float* wdehalo;
@@ -45,26 +57,20 @@ struct BlockJob {
   float sharpen, smin_n, smax_n, dehalo, ht2n, kratio2;
   int use_pattern, use_degrid;
   // spectra, layout [block][bh][outwidth] float2 (compact)
-  float2* spec_out;          // MODE_FWD destination
-  const float2* spec_in[5];  // MODE_W3D temporal inputs, oldest first
-  const float2* spec_cur;    // the current frame's spectrum (degrid DC)
+  float2* spec_out;          // where this launch's forward spectra are stored (or null)
+  const float2* spec_in[5];  // Wiener-3D window, oldest first (on-chip newest: unused)
   float2* kal_last;
   float2* kal_covar;
   float2* kal_cproc;
-  // synthesised blocks, layout [block][bh][bw] float
-  float* blocks;
 };
 
-// Synthesis overlap-add of one plane (src/buffer.cpp:381-560 + 82-116).
-struct OlaJob {
-  int nox, noy, bw, bh, ow, oh, mirw, mirh, Wc, Hc, interlaced;
-  const float* blocks;
-  const float* wsynxl; const float* wsynxr; const float* wsynyl; const float* wsynyr;
-  float norm;
-  uint8_t* dst; long long dst_pitch; int lc, tc;
-  int pix;        // 1, 2, 4
-  int base;       // chroma planeBase (integer formats)
-  int maxval;     // 2^bits - 1 (integer formats)
+// One launch: up to three planes of one frame (same block shape).
+struct TileLaunch {
+  PlaneJob job[3];
+  int njobs;
+  int total_wg;
+  int total_seam_rows;
+  int max_wc;
 };
 
 // Pattern estimation (src/helper.cpp:46-132).
@@ -82,9 +88,10 @@ struct PatternJob {
 
 // Launchers; return hipSuccess or the launch error.  `bw`/`bh` select the
 // compiled block shape; unsupported shapes return hipErrorInvalidValue.
-hipError_t launch_block(int mode, int bt, int bw, int bh, const BlockJob& job, hipStream_t s);
-hipError_t launch_ola(const OlaJob& job, hipStream_t s);
+// launch_tile runs k_tile and, for the filtering modes, k_seam.
+hipError_t launch_tile(int mode, int bt, int onchip, int bw, int bh, const TileLaunch& l, hipStream_t s);
 hipError_t launch_pattern(const PatternJob& job, hipStream_t s);
 bool block_shape_supported(int bw, int bh);
+int tile_blocks_per_wg(int bw, int bh);
 
 }  // namespace nf3d

@@ -18,6 +18,8 @@
 #include "fft3d.hpp"
 
 #include <atomic>
+#include <mutex>
+#include <thread>
 #include <cstdlib>
 #include <map>
 #include <memory>
@@ -264,6 +266,40 @@ int nf3dref_get_frame(void* h, int n, void* const* dst, const int* dst_strides, 
 }
 
 void nf3dref_free(void* h) { delete static_cast<RefClip*>(h); }
+
+// Frames [first, first+count) through the front end from `threads` host
+// threads pulling frame numbers from a shared counter (a VapourSynth core
+// issuing concurrent getFrame calls to an fmParallel filter); outputs go to
+// per-thread scratch frames.  Returns 0, or -1 with the first error.
+int nf3dref_run_parallel(void* h, int first, int count, int threads, char* err, int errlen) {
+  auto* clip = static_cast<RefClip*>(h);
+  std::atomic<int> next{first};
+  std::atomic<int> failed{0};
+  std::mutex err_mu;
+  auto worker = [&]() {
+    VSFrameRef* scratch = alloc_frame(&clip->fmt, clip->width, clip->height);
+    void* dst[4];
+    int ds[4];
+    for (int p = 0; p < clip->fmt.numPlanes; p++) {
+      dst[p] = scratch->wptr[p];
+      ds[p] = scratch->stride[p];
+    }
+    char local[256];
+    for (;;) {
+      int n = next.fetch_add(1);
+      if (n >= first + count || failed.load()) break;
+      if (nf3dref_get_frame(h, n, dst, ds, local, sizeof(local)) != 0) {
+        std::lock_guard<std::mutex> lock(err_mu);
+        if (!failed.exchange(1)) set_err(err, errlen, "", local);
+      }
+    }
+    fake_free(scratch);
+  };
+  std::vector<std::thread> pool;
+  for (int t = 0; t < std::max(1, threads); t++) pool.emplace_back(worker);
+  for (auto& t : pool) t.join();
+  return failed.load() ? -1 : 0;
+}
 
 const char* nf3dref_vs_params() {
   static std::string s = FFT3D().VSParams();

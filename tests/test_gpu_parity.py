@@ -98,3 +98,23 @@ def test_gray8_640x360_bt3_48(product, oracle):
     """The CPU-runnable workload at full size: Gray8 640x360, bt=3, 48/16."""
     check_clip(product, oracle, "gray", 8, False, "sigma=2;bt=3;bw=48;bh=48;ow=16;oh=16;sharpen=0",
                W=640, H=360, F=30, frames=[0, 1, 2, 15, 28, 29])
+
+
+# ------------------------------------------------------------ golden vectors
+from test_oracle import GOLDEN, golden_bar, load_golden  # noqa: E402
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p)[:-4] for p in GOLDEN])
+def test_matches_reference_golden(product, path):
+    """HIP path vs the compiled reference's own outputs (opt=1 C path)."""
+    clip, args, outs = load_golden(path)
+    h = product.open(args, clip)
+    bar = golden_bar(clip)
+    for n, want in enumerate(outs):
+        got = h.get(n)
+        for p, (mx, exact, rms) in enumerate(compare(got, want, clip.is_float)):
+            print(f"{os.path.basename(path)} n={n} p={p} max={mx:.3g} exact={exact:.5f} rms={rms:.3g}", flush=True)
+            if clip.is_float:
+                assert mx <= 1e-4, (n, p, mx)
+            else:
+                assert mx <= 1 and exact >= bar, (n, p, mx, exact)
