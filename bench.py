@@ -115,19 +115,22 @@ def gpu_run(args):
     elapsed = t1 - t0
     fps = F * args.steps / elapsed
 
-    # roofline of the main spectral kernel (HIP events on its stream)
+    # roofline of the filtering kernels (HIP events on each engine's stream):
+    # the frame-loop launches (k_run + k_seam_run, 16 frames each) and the
+    # per-frame launches at the clip ends (k_tile + k_seam); the engines
+    # report the frames their timed launches filtered
     pdims = [(W, H), (W // 2, H // 2), (W // 2, H // 2)]
     bytes_total = 0.0
     ms_total = 0.0
-    launches = 0
+    frames_timed = 0
     for (pw, ph), (ms, n) in zip(pdims, prof):
-        # every timed launch is one frame of one plane (k_tile + k_seam); the 2
-        # edge frames of each pass run the 2D path and are counted alike (<1%)
+        # the 2 edge frames of each pass run the 2D path and are counted at
+        # the 3D bytes (<1% of the frames)
         bytes_total += frame_plane_bytes(pw, ph, args.bt) * n
         ms_total += ms
-        launches += n
+        frames_timed += n
     achieved = bytes_total / (ms_total * 1e-3) / 1e9 if ms_total > 0 else 0.0
-    avg_us = ms_total * 1e3 / max(launches, 1)
+    avg_us = ms_total * 1e3 / max(frames_timed, 1)
 
     # quality: denoised vs a noise-free render of the same clip
     idx = [F // 2 + k for k in range(4)]
@@ -142,8 +145,8 @@ def gpu_run(args):
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": f"k_tile<48,48,MODE_W3D,bt={args.bt},newest on chip> + k_seam",
-            "avg_launch_us": round(avg_us, 2), "launches": launches,
+            "kernel": f"k_run<48,48,MODE_W3D,bt={args.bt},sharpen> + k_seam_run",
+            "us_per_frame_plane": round(avg_us, 2), "frame_planes": frames_timed,
             "bytes_model": "per frame-plane: 2*W*H*Bps + bt*nb*bh*(bw/2+1)*8",
         },
         "psnr_db": q,

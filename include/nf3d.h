@@ -101,12 +101,13 @@ int nf3d_engine_cache_refresh(nf3d_engine* e, int n);
 /* Block geometry of the engine: nox, noy, outwidth, mirw, mirh, ow, oh. */
 int nf3d_engine_geometry(const nf3d_engine* e, int* out7);
 
-/* Diagnostics (no reference counterpart): when enabled, the main spectral
- * kernel of every frame is bracketed by HIP events on its stream;
+/* Diagnostics (no reference counterpart): when enabled, every filtering
+ * launch (the spectral kernel with its seam pass; one frame, or a run of
+ * frames in nf3d_engine_run_device) is bracketed by HIP events on its stream;
  * nf3d_engine_profile_read waits for them and returns the summed kernel time
- * (ms) and the number of timed launches since the last read. */
+ * (ms) and the number of frames those launches filtered since the last read. */
 int nf3d_engine_profile(nf3d_engine* e, int enable);
-int nf3d_engine_profile_read(nf3d_engine* e, double* total_ms, int* launches);
+int nf3d_engine_profile_read(nf3d_engine* e, double* total_ms, int* frames);
 
 /* Wait for the engine's queued work (the host-fed calls already do). */
 int nf3d_engine_sync(nf3d_engine* e);

@@ -56,11 +56,14 @@ def build_product(verbose: bool = False) -> str:
         o = os.path.join(BUILD, u + ".o")
         jobs.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, u), "-o", o])
         objs.append(o)
-    for bw, bh in block_shapes():
-        o = os.path.join(BUILD, f"nf3d_shape_{bw}x{bh}.o")
-        jobs.append([HIPCC, *HIP_FLAGS, f"-DNF3D_BW={bw}", f"-DNF3D_BH={bh}", "-c",
-                     os.path.join(CSRC, "nf3d_shape.hip"), "-o", o])
-        objs.append(o)
+    # each block shape in two units (per-frame kernels, frame-loop kernels)
+    # so the slowest compiles run side by side
+    for bw, bh in sorted(block_shapes(), key=lambda s: -s[0] * s[1]):
+        for part in ("RUN", "TILE"):
+            o = os.path.join(BUILD, f"nf3d_shape_{bw}x{bh}_{part.lower()}.o")
+            jobs.append([HIPCC, *HIP_FLAGS, f"-DNF3D_BW={bw}", f"-DNF3D_BH={bh}", f"-DNF3D_PART_{part}", "-c",
+                         os.path.join(CSRC, "nf3d_shape.hip"), "-o", o])
+            objs.append(o)
     for u in HOST_UNITS:
         o = os.path.join(BUILD, u + ".o")
         jobs.append([HIPCC, *HIP_FLAGS, "-x", "hip", "-c", os.path.join(CSRC, u), "-o", o])

@@ -35,6 +35,7 @@
 
 using nf3d::PatternJob;
 using nf3d::PlaneJob;
+using nf3d::RunLaunch;
 using nf3d::TileLaunch;
 
 namespace {
@@ -104,7 +105,11 @@ struct nf3d_engine {
   // opt-in HIP-event timing of the main spectral kernel of each frame
   bool profiling = false;
   std::vector<hipEvent_t> ev_pool;
+  std::vector<int> ev_frames;  // frames filtered by each timed launch
   size_t ev_used = 0;
+  // frame-loop launches (k_run): per-frame seam buffers
+  float* d_seam_run = nullptr;
+  static constexpr int kRunFrames = 16;  // frames per k_run launch
 
   ~nf3d_engine() {
     hipFree(d_tab);
@@ -114,6 +119,7 @@ struct nf3d_engine {
     hipFree(d_kal_covar);
     hipFree(d_kal_cproc);
     hipFree(d_seam);
+    hipFree(d_seam_run);
     hipFree(d_spec_tmp);
     hipFree(d_psums);
     hipFree(d_choice);
@@ -478,6 +484,21 @@ struct nf3d_engine {
     return Src{d_stage[slot], (long long)stage_pitch};
   }
 
+  // Start / end events of a timed launch (null when profiling is off).
+  std::pair<hipEvent_t, hipEvent_t> timer(bool timed, int nframes, hipStream_t s) {
+    if (!timed || !profiling) return {nullptr, nullptr};
+    while (ev_pool.size() < ev_used + 2) {
+      hipEvent_t ev;
+      check(hipEventCreate(&ev), "hipEventCreate");
+      ev_pool.push_back(ev);
+    }
+    ev_frames.resize(ev_pool.size() / 2);
+    ev_frames[ev_used / 2] = nframes;
+    hipEvent_t a = ev_pool[ev_used++], b = ev_pool[ev_used++];
+    check(hipEventRecord(a, s), "hipEventRecord");
+    return {a, b};
+  }
+
   void launch(int mode, int bt, const PlaneJob& J, hipStream_t s, bool timed = false, int onchip = 0) {
     TileLaunch tl{};
     tl.job[0] = J;
@@ -485,31 +506,24 @@ struct nf3d_engine {
     tl.total_wg = noy * ntx;
     tl.total_seam_rows = Hc;
     tl.max_wc = Wc;
-    hipEvent_t a = nullptr, b = nullptr;
-    if (timed && profiling) {
-      while (ev_pool.size() < ev_used + 2) {
-        hipEvent_t ev;
-        check(hipEventCreate(&ev), "hipEventCreate");
-        ev_pool.push_back(ev);
-      }
-      a = ev_pool[ev_used++];
-      b = ev_pool[ev_used++];
-      check(hipEventRecord(a, s), "hipEventRecord");
-    }
+    auto ev = timer(timed, 1, s);
     check(nf3d::launch_tile(mode, bt, onchip, ep.bw, ep.bh, tl, s), "tile kernel");
-    if (b) check(hipEventRecord(b, s), "hipEventRecord");
+    if (ev.second) check(hipEventRecord(ev.second, s), "hipEventRecord");
   }
 
-  // Sum of the timed launches' durations (ms) since the last read.
-  double profile_read(int* launches) {
+  // Sum of the timed launches' durations (ms) and the frames they filtered
+  // since the last read.
+  double profile_read(int* nframes) {
     double total = 0;
+    int n = 0;
     for (size_t i = 0; i + 1 < ev_used; i += 2) {
       check(hipEventSynchronize(ev_pool[i + 1]), "hipEventSynchronize");
       float ms = 0;
       check(hipEventElapsedTime(&ms, ev_pool[i], ev_pool[i + 1]), "hipEventElapsedTime");
       total += ms;
+      n += ev_frames[i / 2];
     }
-    if (launches) *launches = (int)(ev_used / 2);
+    if (nframes) *nframes = n;
     ev_used = 0;
     return total;
   }
@@ -650,6 +664,111 @@ struct nf3d_engine {
     }
   }
 
+  // One frame of a device-resident clip through the per-frame path, with
+  // its temporal window (and the pattern frame while the pattern is pending).
+  void run_one(int n, const void* const* src, long long src_pitch, void* const* dst, long long dst_pitch,
+               hipStream_t s) {
+    nf3d_frame_ref win[6];
+    int k = 0;
+    const int lo = ep.bt > 1 ? std::max(n - ep.bt / 2, 0) : n;
+    const int hi = ep.bt > 1 ? std::min(n + (ep.bt - 1) / 2, frames - 1) : n;
+    for (int i = lo; i <= hi; i++) win[k++] = nf3d_frame_ref{i, src[i], (int)src_pitch};
+    if (ep.pfactor != 0 && !pattern_set) {
+      const int pf = std::max(0, std::min(ep.pframe, frames - 1));
+      if (pf < lo || pf > hi) win[k++] = nf3d_frame_ref{pf, src[pf], (int)src_pitch};
+    }
+    Call c{win, k, true, s};
+    run(n, c, static_cast<uint8_t*>(dst[n]), dst_pitch);
+  }
+
+  int shp_kind() const {
+    if (ep.sharpen == 0.0f && ep.dehalo == 0.0f) return nf3d::SHP_NONE;
+    return ep.dehalo == 0.0f ? nf3d::SHP_SHARPEN : nf3d::SHP_GENERAL;
+  }
+
+  // Frames [first, first + count) of a device-resident clip, in order.  The
+  // frames whose filter window is whole (every frame for bt = 1 / -1, all but
+  // the first for Kalman, btcur == bt for Wiener 3D) go through k_run, up to
+  // kRunFrames per launch; the others (clip ends) through the per-frame path.
+  void run_range(int first, int count, const void* const* src, long long src_pitch, void* const* dst,
+                 long long dst_pitch, hipStream_t s) {
+    check(hipSetDevice(device), "hipSetDevice");
+    for (int n = first; n < first + count; n++)
+      if (n < 0 || n >= frames) fail("frame " + std::to_string(n) + " outside the clip");
+    int lo = 0, hi = frames - 1;
+    int mode = nf3d::MODE_W2D;
+    if (ep.bt >= 2) {
+      mode = nf3d::MODE_W3D;
+      lo = ep.bt / 2;
+      hi = frames - 1 - (ep.bt - 1) / 2;
+    } else if (ep.bt == 0) {
+      mode = nf3d::MODE_KALMAN;
+      lo = 1;
+    } else if (ep.bt == -1) {
+      mode = nf3d::MODE_SHARPEN;
+    }
+    const int shp = shp_kind();
+    const int a = std::max(first, lo), b = std::min(first + count - 1, hi);
+    // k_run keeps 32-bit row offsets; the sharpen-only mode without sharpen
+    // or dehalo is a plain round trip left to the per-frame path
+    const bool fits = (long long)(tc + Hc) * src_pitch < (1LL << 31);
+    if (a > b || !fits || (mode == nf3d::MODE_SHARPEN && shp == nf3d::SHP_NONE)) {
+      for (int n = first; n < first + count; n++) run_one(n, src, src_pitch, dst, dst_pitch, s);
+      return;
+    }
+    for (int n = first; n < a; n++) run_one(n, src, src_pitch, dst, dst_pitch, s);
+    if (ep.pfactor != 0 && !pattern_set) {
+      const int pf = std::max(0, std::min(ep.pframe, frames - 1));
+      nf3d_frame_ref r{pf, src[pf], (int)src_pitch};
+      estimate_pattern(Call{&r, 1, true, s});
+    }
+    const int from = mode == nf3d::MODE_W3D ? -(ep.bt / 2) : 0;
+    const int to = mode == nf3d::MODE_W3D ? (ep.bt - 1) / 2 : 0;
+    if (mode == nf3d::MODE_W3D) {
+      if (ep.pfactor != 0) ensure_pattern3d(ep.bt, s);
+      // the window's older frames of the first step, into slot (frame % bt)
+      for (int k = a + from; k < a + to; k++) {
+        PlaneJob F = base_job();
+        F.src = static_cast<const uint8_t*>(src[k]);
+        F.src_pitch = src_pitch;
+        F.spec_out = slot_ptr(k % ep.bt);
+        launch(nf3d::MODE_FWD, 0, F, s);
+      }
+    }
+    if (!d_seam_run) d_seam_run = dev_alloc<float>((size_t)kRunFrames * 4 * Hc * Wc);
+    for (int m0 = a; m0 <= b; m0 += kRunFrames) {
+      const int nf = std::min(kRunFrames, b - m0 + 1);
+      RunLaunch R{};
+      R.J = base_job();
+      R.J.sigma2 = (float)(mode == nf3d::MODE_W3D ? ep.bt : 1) * ep.sigma * ep.sigma / norm;
+      R.J.seam = d_seam_run;
+      R.J.src_pitch = src_pitch;
+      R.J.dst_pitch = dst_pitch;
+      for (int i = 0; i < nf; i++) {
+        R.src[i] = static_cast<const uint8_t*>(src[m0 + i + to]);
+        R.dst[i] = static_cast<uint8_t*>(dst[m0 + i]);
+      }
+      R.ring = d_ring;
+      R.spec_elems = (long long)spec_elems;
+      R.m0 = m0;
+      R.nframes = nf;
+      R.fpw = (mode == nf3d::MODE_W3D || mode == nf3d::MODE_KALMAN) ? nf : 1;
+      R.ntiles = noy * ntx;
+      auto ev = timer(true, nf, s);
+      check(nf3d::launch_run(mode, ep.bt, shp, ep.bw, ep.bh, R, s), "frame-loop kernel");
+      if (ev.second) check(hipEventRecord(ev.second, s), "hipEventRecord");
+    }
+    if (mode == nf3d::MODE_W3D) {
+      // slot (k % bt) now holds frame k for the last bt frames transformed
+      std::fill(ring_frame.begin(), ring_frame.end(), -1);
+      for (int k = std::max(0, b + to - ep.bt + 1); k <= b + to; k++) {
+        ring_frame[k % ep.bt] = k;
+        ring_tick[k % ep.bt] = ++tick;
+      }
+    }
+    for (int n = b + 1; n < first + count; n++) run_one(n, src, src_pitch, dst, dst_pitch, s);
+  }
+
   void get_frame_host(int n, const nf3d_frame_ref* frames_, int nframes, void* dst, int dst_stride) {
     Call c{frames_, nframes, false, stream};
     std::fill(stage_frame.begin(), stage_frame.end(), -1);
@@ -751,20 +870,7 @@ int nf3d_engine_run_device(nf3d_engine* e, int first, int count, const void* con
   std::lock_guard<std::mutex> lock(e->mu);
   try {
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : e->stream;
-    for (int n = first; n < first + count; n++) {
-      if (n < 0 || n >= e->frames) fail("frame " + std::to_string(n) + " outside the clip");
-      nf3d_frame_ref win[6];
-      int k = 0;
-      int lo = e->ep.bt > 1 ? std::max(n - e->ep.bt / 2, 0) : n;
-      int hi = e->ep.bt > 1 ? std::min(n + (e->ep.bt - 1) / 2, e->frames - 1) : n;
-      for (int i = lo; i <= hi; i++) win[k++] = nf3d_frame_ref{i, src[i], src_stride};
-      if (e->ep.pfactor != 0 && !e->pattern_set) {
-        int pf = std::max(0, std::min(e->ep.pframe, e->frames - 1));
-        if (pf < lo || pf > hi) win[k++] = nf3d_frame_ref{pf, src[pf], src_stride};
-      }
-      nf3d_engine::Call c{win, k, true, s};
-      e->run(n, c, static_cast<uint8_t*>(dst[n]), dst_stride);
-    }
+    e->run_range(first, count, src, src_stride, dst, dst_stride, s);
   } catch (const Failure& f) {
     copy_err(err, errlen, f.msg);
     return -1;
@@ -796,10 +902,10 @@ int nf3d_engine_profile(nf3d_engine* e, int enable) {
   return 0;
 }
 
-int nf3d_engine_profile_read(nf3d_engine* e, double* total_ms, int* launches) {
+int nf3d_engine_profile_read(nf3d_engine* e, double* total_ms, int* frames) {
   std::lock_guard<std::mutex> lock(e->mu);
   try {
-    *total_ms = e->profile_read(launches);
+    *total_ms = e->profile_read(frames);
   } catch (const Failure&) {
     return -1;
   }

@@ -73,6 +73,28 @@ struct TileLaunch {
   int max_wc;
 };
 
+// Frame-loop launch (k_run): one plane, up to RUN_MAX_FRAMES consecutive
+// frames, every tile of the plane.
+constexpr int RUN_MAX_FRAMES = 32;
+
+// Specialisation of the sharpen / dehalo factors in k_run's spectral pass.
+enum Shp : int {
+  SHP_NONE = 0,     // sharpen == 0 and dehalo == 0
+  SHP_SHARPEN = 1,  // dehalo == 0
+  SHP_GENERAL = 3,  // dehalo != 0 (sharpen factor times dehalo factor; 1 exactly when sharpen == 0)
+};
+
+struct RunLaunch {
+  PlaneJob J;                           // geometry, tables, parameters; J.seam = per-frame seam buffers
+  const uint8_t* src[RUN_MAX_FRAMES];   // plane transformed on chip at step i (frame m0+i, +(bt-1)/2 for Wiener 3D)
+  uint8_t* dst[RUN_MAX_FRAMES];         // output plane of frame m0+i
+  float2* ring;                         // Wiener 3D: bt slots of spec_elems, frame k in slot k % bt
+  long long spec_elems;
+  int m0, nframes;
+  int fpw;                              // consecutive frames filtered by one workgroup
+  int ntiles;                           // tiles of the plane (noy * ntx)
+};
+
 // Pattern estimation (src/helper.cpp:46-132).
 struct PatternJob {
   int nox, noy, bh, outwidth;
@@ -91,6 +113,8 @@ struct PatternJob {
 // launch_tile runs k_tile and, for the filtering modes, k_seam.
 hipError_t launch_tile(int mode, int bt, int onchip, int bw, int bh, const TileLaunch& l, hipStream_t s);
 hipError_t launch_pattern(const PatternJob& job, hipStream_t s);
+// k_run over L.nframes frames, then k_seam_run; `shp` is a Shp value.
+hipError_t launch_run(int mode, int bt, int shp, int bw, int bh, const RunLaunch& l, hipStream_t s);
 bool block_shape_supported(int bw, int bh);
 int tile_blocks_per_wg(int bw, int bh);
 

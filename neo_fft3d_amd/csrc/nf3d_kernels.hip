@@ -63,7 +63,9 @@ __global__ void __launch_bounds__(256) k_pattern_set(PatternJob J) {
 // Block shapes compiled for k_tile (one translation unit each).
 #define NF3D_SHAPES(X) X(8, 8) X(16, 16) X(24, 24) X(32, 32) X(48, 48) X(64, 64) X(32, 16) X(16, 32)
 
-#define X(W, H) hipError_t launch_tile_##W##x##H(int mode, int bt, int onchip, const TileLaunch& l, hipStream_t s);
+#define X(W, H)                                                                                          \
+  hipError_t launch_tile_##W##x##H(int mode, int bt, int onchip, const TileLaunch& l, hipStream_t s); \
+  hipError_t launch_run_##W##x##H(int mode, int bt, int shp, const RunLaunch& l, hipStream_t s);
 NF3D_SHAPES(X)
 #undef X
 
@@ -82,6 +84,13 @@ int tile_blocks_per_wg(int bw, int bh) {
 
 hipError_t launch_tile(int mode, int bt, int onchip, int bw, int bh, const TileLaunch& l, hipStream_t s) {
 #define X(W, H) if (bw == W && bh == H) return launch_tile_##W##x##H(mode, bt, onchip, l, s);
+  NF3D_SHAPES(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_run(int mode, int bt, int shp, int bw, int bh, const RunLaunch& l, hipStream_t s) {
+#define X(W, H) if (bw == W && bh == H) return launch_run_##W##x##H(mode, bt, shp, l, s);
   NF3D_SHAPES(X)
 #undef X
   return hipErrorInvalidValue;

@@ -1,0 +1,626 @@
+// Frame-loop kernel of the denoiser: k_run<BW,BH,NT,MODE,BT,SHP>.
+//
+// A workgroup owns one tile (up to BPW horizontally adjacent blocks of one
+// block row, the tile of k_tile) and filters it for a run of consecutive
+// frames, so one launch covers many frames and no workgroup waits for the
+// others between frames.  Per frame the phases are those of k_tile (gather,
+// row r2c, column FFT, spectral pass, column inverse, row c2r, overlap-add);
+// what changes is where the data waits are:
+//   - the pixels of the next frame are loaded into registers during the
+//     current frame's overlap-add, so the gather finds them resident;
+//   - the spectral pass runs down each column in chunks of rows, the ring
+//     spectra, Kalman state and table rows of chunk k+1 loaded while chunk k
+//     is computed; chunk 0 is loaded before the forward column FFT;
+//   - the gains are specialised at compile time on the sharpen / dehalo case
+//     (SHP) and degrid is branch-free (a zero fraction when it is off).
+// Wiener 3D: the newest frame of each window (frame m + (BT-1)/2) is
+// transformed on chip and stored to ring slot (frame % BT); the BT-1 older
+// spectra are the ones the same lanes stored in earlier iterations (or the
+// launches before), so the loop needs no synchronisation beyond the
+// workgroup.  Kalman state likewise stays with its tile.  Seam partials go to
+// a per-frame seam buffer that k_seam_run finishes after the launch.
+#pragma once
+#include "nf3d_tile.h"
+
+namespace nf3d {
+
+__device__ __forceinline__ const RunLaunch& kernarg_run(const RunLaunch& arg) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  (void)arg;
+  return *(const RunLaunch*)__builtin_amdgcn_kernarg_segment_ptr();
+#else
+  return arg;
+#endif
+}
+
+// The same record through an offset the compiler cannot see through (zero at
+// run time): loads of launch fields inside a loop stay in the loop instead of
+// being hoisted into registers that live across it.
+__device__ __forceinline__ const RunLaunch& kernarg_run_opaque(const RunLaunch& arg) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  (void)arg;
+  int zero = 0;
+  asm volatile("" : "+s"(zero));
+  return *(const RunLaunch*)((const char*)__builtin_amdgcn_kernarg_segment_ptr() + zero);
+#else
+  return arg;
+#endif
+}
+
+template <int I, typename X>
+__device__ __forceinline__ X& pick(X& a, X& b) {
+  if constexpr (I == 0) return a;
+  else return b;
+}
+
+// ------------------------------------------------------------------ gather
+// Raw samples of one lane's (up to) two tile columns, all BH rows, kept as
+// bits so no use waits on them before the windowing pass.  Both halves are
+// always written (zeros when the tile is one column pass wide) so no old
+// values stay live around the frame loop.
+template <int PIX, int BH>
+__device__ __forceinline__ void gather_issue(const uint8_t* src, const int* rowoff, int sx0, int sx1, bool two,
+                                             uint32_t (&raw)[2][BH]) {
+  using T = typename std::conditional<PIX == 1, uint8_t, typename std::conditional<PIX == 2, uint16_t, uint32_t>::type>::type;
+  static_for<0, BH>([&](auto rr) {
+    const T* row = reinterpret_cast<const T*>(src + rowoff[decltype(rr)::value]);
+    raw[0][rr] = (uint32_t)row[sx0];
+  });
+  if (two) {
+    static_for<0, BH>([&](auto rr) {
+      const T* row = reinterpret_cast<const T*>(src + rowoff[decltype(rr)::value]);
+      raw[1][rr] = (uint32_t)row[sx1];
+    });
+  } else {
+    static_for<0, BH>([&](auto rr) { raw[1][rr] = 0u; });
+  }
+}
+
+// Windowing of the loaded columns into per-block row pairs: the product
+// order of gather_tile ((wx*wy)*p on the outer corners of the edge blocks,
+// (wy*p)*wx on the other windowed columns of the first/last block rows,
+// (wx*p)*wy elsewhere; src/buffer.cpp:137-378).  A column held by one block
+// writes its sample twice to the same place, so no lane branches per row.
+template <int PIX, int BW, int BH, int NT, int SA, int LA>
+__device__ __forceinline__ void gather_window(const PlaneJob& J, float2* stage, const uint32_t (&raw)[2][BH],
+                                              const float* winx, const float* winy, int by, int bx0, int nblk,
+                                              int RW, int stepx, int t) {
+  const int ow = J.ow, oh = J.oh;
+  static_for<0, 2>([&](auto pp) {
+    constexpr int p = decltype(pp)::value;
+    if (p == 1 && RW <= NT) return;  // uniform
+    const int c0 = t + p * NT;
+    const bool cv = c0 < RW;
+    const int c = cv ? c0 : RW - 1;
+    int bA = c / stepx;
+    if (bA >= nblk) bA = nblk - 1;
+    const int jA = c - bA * stepx;
+    const bool hasB = bA >= 1 && jA + stepx < BW;
+    const int bB = hasB ? bA - 1 : bA, jB = hasB ? jA + stepx : jA;
+    auto xinfo = [&](int b, int j, float& wx, int& side, bool& xedge) {
+      side = j < ow ? 1 : (j >= BW - ow ? 2 : 0);
+      wx = winx[j];
+      const int bx = bx0 + b;
+      xedge = (bx == 0 && side == 1) || (bx == J.nox - 1 && side == 2);
+    };
+    float wxA, wxB;
+    int sA, sB;
+    bool eA, eB;
+    xinfo(bA, jA, wxA, sA, eA);
+    xinfo(bB, jB, wxB, sB, eB);
+    float* dA = reinterpret_cast<float*>(&stage[bA * LA + jA]);
+    float* dB = reinterpret_cast<float*>(&stage[bB * LA + jB]);
+    if (!cv) return;  // tail lanes: nothing to store
+    static_for<0, BH>([&](auto rr) {
+      constexpr int i = decltype(rr)::value;
+      float v;
+      if constexpr (PIX == 4) v = __uint_as_float(raw[p][i]);
+      else v = (float)raw[p][i];
+      const float px = v - J.base;
+      const bool ytop = i < oh, ybot = i >= BH - oh;
+      const bool yw = ytop || ybot;
+      const bool yedge = (by == 0 && ytop) || (by == J.noy - 1 && ybot);
+      const float wy = winy[i];
+      auto windowed = [&](float wx, int side, bool xedge) {
+        float a, b;
+        if (side != 0 && yw) {
+          a = yedge ? (xedge ? wx * wy : wy) : wx;
+          b = yedge ? (xedge ? 1.0f : wx) : wy;
+        } else {
+          a = side != 0 ? wx : (yw ? wy : 1.0f);
+          b = 1.0f;
+        }
+        return (a * px) * b;
+      };
+      constexpr int off = (i >> 1) * SA * 2 + (i & 1);
+      dA[off] = windowed(wxA, sA, eA);
+      dB[off] = windowed(wxB, sB, eB);
+    });
+  });
+}
+
+// Phase 7 for k_run: overlap-add of the tile in the reference's float order
+// (src/buffer.cpp:381-560), as overlap_tile; lanes compute every row and only
+// the stores are predicated.
+template <int PIX, int BW, int BH, int NT, int LSYN, int SYNW>
+__device__ __forceinline__ void overlap_run(const PlaneJob& J, uint8_t* dst, float* seam, const float* syn, int by,
+                                            int bx0, int nblk, int RW, int stepx, int stepy, int t) {
+  using T = typename std::conditional<PIX == 1, uint8_t, typename std::conditional<PIX == 2, uint16_t, float>::type>::type;
+  const float rounder = PIX == 4 ? 0.0f : 0.5f;
+  const float fbase = (float)J.obase;
+  const int cy0 = by * stepy, cx0 = bx0 * stepx;
+  const size_t P = (size_t)J.seam_plane;
+  static_for<0, 2>([&](auto pp) {
+    constexpr int p = decltype(pp)::value;
+    if (p == 1 && RW <= NT) return;  // uniform
+    const int xl = t + p * NT;
+    const int x = cx0 + xl - J.mirw;
+    const bool xin = xl < RW && x >= 0 && x < J.Wc;
+    // x class: 0 exclusive column, 1 band inside the tile, 2 band shared with
+    // the left tile (this tile holds its right block), 3 band shared with the
+    // right tile (this tile holds its left block)
+    int cls = 0, la = 0, lb = 0;
+    float wa = 0.0f, wb = 1.0f;
+    if (xl < nblk * stepx) {
+      const int bl = xl / stepx, j = xl - bl * stepx;
+      lb = bl * LSYN + j;
+      la = lb;
+      if (j < J.ow) {
+        wb = J.wsynxl[j];
+        if (bl >= 1) {
+          cls = 1;
+          la = (bl - 1) * LSYN + j + stepx;
+          wa = J.wsynxr[j];
+        } else {
+          cls = 2;
+        }
+      }
+    } else if (xl < RW) {
+      const int jj = xl - nblk * stepx;
+      cls = 3;
+      la = (nblk - 1) * LSYN + stepx + jj;
+      lb = la;
+      wa = J.wsynxr[jj];
+      wb = 0.0f;
+    }
+    const bool own = xin && cls <= 1;   // pixel finished in this tile (body rows)
+    const bool shr = xin && cls >= 2;   // column seam (body rows)
+    const size_t slot_x = (size_t)(cls == 3 ? 0 : 1) * P;
+    T* dcol = reinterpret_cast<T*>(dst + (long long)J.tc * J.dst_pitch) + x + J.lc;
+#pragma unroll 2
+    for (int i = 0; i < BH; i++) {
+      const int ry = cy0 + i - J.mirh;
+      if (ry < 0 || ry >= J.Hc) continue;  // padding rows (uniform)
+      const float h = syn[la + i * SYNW] * wa + syn[lb + i * SYNW] * wb;
+      const int yo = interior_to_out_row(J, ry);
+      const size_t so = (size_t)yo * J.Wc + x;
+      if (i >= J.oh && i < stepy) {  // rows of this block row only (uniform)
+        const float v = h * J.norm + rounder + fbase;
+        T q;
+        if constexpr (PIX == 4) {
+          const float o = v < 0.0f ? 0.0f : v;
+          q = o > 1.0f ? 1.0f : o;
+        } else {
+          int qi = (int)v;
+          qi = qi < 0 ? 0 : qi;
+          q = (T)(qi > J.maxval ? J.maxval : qi);
+        }
+        if (own) *reinterpret_cast<T*>(reinterpret_cast<uint8_t*>(dcol) + (long long)yo * J.dst_pitch) = q;
+        if (shr) seam[slot_x + so] = h;  // slot 0 = left block's part, slot 1 = right's
+      } else {
+        const bool ytop = i < J.oh;
+        // row seam: slot 0 = upper block row's part, slot 1 = lower's (times
+        // its window); corner: A upper-left, B upper-right, C lower-left,
+        // D lower-right
+        const float w = ytop ? J.wsynyl[i] * J.norm : J.wsynyr[i - stepy] * J.norm;
+        const int slot = cls <= 1 ? (ytop ? 1 : 0) : (ytop ? (cls == 3 ? 2 : 3) : (cls == 3 ? 0 : 1));
+        const float val = cls <= 1 ? h * w : h;
+        if (xin) seam[(size_t)slot * P + so] = val;
+      }
+    }
+  });
+}
+
+// ------------------------------------------------------------ spectral pass
+// Sharpen / dehalo factor of one element (src/code_impl/Sharpen.cpp:102-105);
+// for SHP_GENERAL the sharpen factor is exactly 1 when sharpen == 0, so the
+// product equals the reference's dehalo-only factor.
+template <int SHP>
+__device__ __forceinline__ float shp_factor(const PlaneJob& J, float psd, float wsh, float wdh) {
+  if constexpr (SHP == SHP_SHARPEN) return sharpen_factor(J, psd, wsh);
+  else return sharpen_factor(J, psd, wsh) * dehalo_factor(J, psd, wdh);
+}
+
+// Sharpen pass of one element (src/code_impl/Sharpen.cpp:13-133); degrid
+// fraction zero when degrid is off.
+template <int SHP>
+__device__ __forceinline__ float2 sharpen_s(const PlaneJob& J, float2 x, const Tab& T, float gf) {
+  const float g0 = gf * T.g.x, g1 = gf * T.g.y;
+  const float cr = x.x - g0, ci = x.y - g1;
+  const float psd = cr * cr + ci * ci + 1.0e-15f;
+  const float f = shp_factor<SHP>(J, psd, T.wsh, T.wdh);
+  return make_float2(cr * f + g0, ci * f + g1);
+}
+
+// Apply2D (src/code_impl/Apply.cpp:13-65) with the sharpen factor folded in
+// when no pattern is used.
+template <int SHP>
+__device__ __forceinline__ float2 apply2d_s(const PlaneJob& J, float2 x, const Tab& T, float gf) {
+  const float g0 = gf * T.g.x, g1 = gf * T.g.y;
+  const float cr = x.x - g0, ci = x.y - g1;
+  const float psd = cr * cr + ci * ci + 1e-15f;
+  float f = fast_div(psd - T.sig, psd);
+  f = f > J.lowlimit ? f : J.lowlimit;
+  if constexpr (SHP != SHP_NONE) {
+    if (!J.use_pattern) f *= shp_factor<SHP>(J, psd, T.wsh, T.wdh);
+  }
+  return make_float2(cr * f + g0, ci * f + g1);
+}
+
+// One chunk of CH rows of a lane's column: ring spectra, Kalman state and
+// table values, loaded together.
+template <int MODE, int NQ, int SHP, int CH>
+struct ChunkBuf {
+  float2 v[NQ > 0 ? NQ : 1][CH];
+  KalState ks[MODE == MODE_KALMAN ? CH : 1];
+  float2 g[CH];
+  float sig[CH];
+  float wsh[SHP != SHP_NONE ? CH : 1];
+  float wdh[SHP == SHP_GENERAL ? CH : 1];
+};
+
+// ------------------------------------------------------------------ kernel
+// Register arrays (row and column transforms, chunk buffers) are computed by
+// every lane, idle lanes on a duplicate of a valid column (indices clamped to
+// the tile), and only stores are predicated: a register written under a lane
+// mask keeps its old value in the other lanes, which would hold it live around
+// the frame loop.
+template <int BW, int BH, int NT, int MODE, int BT, int SHP>
+__global__ void __launch_bounds__(NT, 2) k_run(RunLaunch L_arg) {
+  const RunLaunch& L0 = kernarg_run(L_arg);
+  using S = TShape<BW, BH, NT>;
+  constexpr int COLS = S::COLS, PAIRS = S::PAIRS, BPW = S::BPW, SR = S::SR, LSPEC = S::LSPEC;
+  constexpr int SYNW = S::SYNW, LSYN = S::LSYN, SA = S::SA, LA = S::LA;
+  constexpr int FROM = MODE == MODE_W3D ? -(BT / 2) : 0;
+  constexpr int TO = MODE == MODE_W3D ? (BT - 1) / 2 : 0;
+  constexpr int NQ = MODE == MODE_W3D ? BT - 1 : 0;  // ring frames per element
+  constexpr int CH = (MODE == MODE_W3D) ? 4 : 2;  // rows per chunk (two chunks of loads in flight)
+  constexpr int NCH = BH / CH;
+  static_assert(BH % CH == 0 && BH % 4 == 0 && BW % 4 == 0, "block shape");
+  static_assert(MODE != MODE_FWD, "k_run filters");
+  using Buf = ChunkBuf<MODE, NQ, SHP, CH>;
+  __shared__ __align__(16) unsigned char smem[S::BYTES];
+  __shared__ float s_gf[BPW], s_gfs[BPW];
+  __shared__ float winx[BW], winy[BH];
+  __shared__ int rowoff[BH];  // byte offset of the source row of each tile row (< 2^31: checked by the host)
+  float2* spec = reinterpret_cast<float2*>(smem);
+  float2* stage = reinterpret_cast<float2*>(smem);
+  float* syn = reinterpret_cast<float*>(smem);
+
+  // ---- tile and frames of this workgroup
+  const int chunk = blockIdx.x / L0.ntiles;
+  const int tile = blockIdx.x - chunk * L0.ntiles;
+  const int i0 = chunk * L0.fpw;
+  const int i1 = min(L0.nframes, i0 + L0.fpw);
+  const int by = tile / L0.J.ntx;
+  const int bx0 = (tile - by * L0.J.ntx) * BPW;
+  const int nblk = min(BPW, L0.J.nox - bx0);
+  const int stepx = BW - L0.J.ow, stepy = BH - L0.J.oh;
+  const int RW = (nblk - 1) * stepx + BW;
+  const bool two = RW > NT;
+
+  {
+    const PlaneJob& J = L0.J;
+    const int t = threadIdx.x;
+    for (int i = t; i < BW; i += NT) winx[i] = J.wanx[i];
+    for (int i = t; i < BH; i += NT) {
+      winy[i] = J.wany[i];
+      rowoff[i] = (int)((long long)(cover_to_src_row(J, by * stepy + i) + J.tc) * J.src_pitch);
+    }
+  }
+  __syncthreads();
+
+  // pixels of a frame's on-chip transform, loaded one frame ahead
+  uint32_t raw[2][BH];
+  auto issue = [&](const PlaneJob& J, const uint8_t* src, const int* ro, int t) {
+    const int sx0 = reflect101(bx0 * stepx + min(t, RW - 1) - J.mirw, J.Wc) + J.lc;
+    const int sx1 = reflect101(bx0 * stepx + min(t + NT, RW - 1) - J.mirw, J.Wc) + J.lc;
+    if (J.pix == 1) gather_issue<1, BH>(src, ro, sx0, sx1, two, raw);
+    else if (J.pix == 2) gather_issue<2, BH>(src, ro, sx0, sx1, two, raw);
+    else gather_issue<4, BH>(src, ro, sx0, sx1, two, raw);
+  };
+  issue(L0.J, L0.src[i0], rowoff, (int)threadIdx.x);
+
+#pragma unroll 1
+  for (int fi = i0; fi < i1; fi++) {
+    // The lane index, the launch record and the LDS tables are reached
+    // through values the compiler cannot see through, so lane addresses and
+    // launch fields are recomputed inside the loop instead of hundreds of
+    // loop-invariant values being kept live across it.
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    int zo = 0;
+    asm volatile("" : "+s"(zo));
+    const RunLaunch& L = kernarg_run_opaque(L_arg);
+    const PlaneJob& J = L.J;
+    const float* wx = winx + zo;
+    const float* wy = winy + zo;
+    const int* ro = rowoff + zo;
+    const int m = L.m0 + fi;
+    const int pix = J.pix;
+    const int cbr = t / COLS, cc = t - cbr * COLS;
+    const bool col_on = cbr < nblk;
+    const int cb = col_on ? cbr : nblk - 1;
+    const int rbr = t / PAIRS, rp = t - rbr * PAIRS;
+    const bool row_on = rbr < nblk;
+    const int rb = row_on ? rbr : nblk - 1;
+    const size_t tile_spec0 = (size_t)(by * J.nox + bx0) * (BH * COLS);
+    const size_t colo = tile_spec0 + (size_t)cb * (BH * COLS) + cc;  // element (0, cc) of this lane's column
+    const float* pat = J.use_pattern ? (MODE == MODE_W3D ? J.pattern3d : J.pattern2d) : nullptr;
+    const float grid0 = J.grid[0].x;
+
+    // ---- 1. window the loaded pixels into per-block row pairs
+    if (pix == 1) gather_window<1, BW, BH, NT, SA, LA>(J, stage, raw, wx, wy, by, bx0, nblk, RW, stepx, t);
+    else if (pix == 2) gather_window<2, BW, BH, NT, SA, LA>(J, stage, raw, wx, wy, by, bx0, nblk, RW, stepx, t);
+    else gather_window<4, BW, BH, NT, SA, LA>(J, stage, raw, wx, wy, by, bx0, nblk, RW, stepx, t);
+    __syncthreads();
+    // ---- 2. row r2c, two real rows per lane
+    {
+      float2 z[BW];
+      static_for<0, BW>([&](auto j) { z[j] = stage[rb * LA + rp * SA + decltype(j)::value]; });
+      __syncthreads();  // gather stage dead; spectral layout from here
+      fft<BW, -1>(z);
+      if (row_on) {
+        float2* dst = spec + rb * LSPEC + 2 * rp * SR;
+        static_for<0, COLS>([&](auto kk) {
+          constexpr int k = decltype(kk)::value;
+          constexpr int km = (BW - k) % BW;
+          const float2 a = z[k], b = z[km];
+          dst[k] = make_float2((a.x + b.x) * 0.5f, (a.y - b.y) * 0.5f);
+          dst[SR + k] = make_float2((a.y + b.y) * 0.5f, (b.x - a.x) * 0.5f);
+        });
+      }
+    }
+    __syncthreads();
+
+    // ---- 3. column lanes: chunk 0 in flight during the forward column FFT
+    const float2* rq[NQ > 0 ? NQ : 1];
+    static_for<0, NQ>([&](auto qq) {
+      constexpr int q = decltype(qq)::value;
+      rq[q] = L.ring + (size_t)((m + FROM + q) % BT) * L.spec_elems;
+    });
+    auto load_chunk = [&](auto kc, Buf& b) {
+      constexpr int k = decltype(kc)::value;
+      static_for<0, CH>([&](auto uu) {
+        constexpr int u = decltype(uu)::value, i = k * CH + u;
+        const size_t o = colo + (size_t)i * COLS;
+        const int kk = i * COLS + cc;
+        static_for<0, NQ>([&](auto qq) { b.v[decltype(qq)::value][u] = rq[decltype(qq)::value][o]; });
+        if constexpr (MODE == MODE_KALMAN) b.ks[u] = KalState{J.kal_last[o], J.kal_covar[o], J.kal_cproc[o]};
+        b.g[u] = J.grid[kk];
+        if constexpr (SHP != SHP_NONE) b.wsh[u] = J.wsharpen[kk];
+        if constexpr (SHP == SHP_GENERAL) b.wdh[u] = J.wdehalo[kk];
+      });
+      if (pat) {
+        static_for<0, CH>([&](auto uu) { b.sig[decltype(uu)::value] = pat[(k * CH + decltype(uu)::value) * COLS + cc]; });
+      } else {
+        static_for<0, CH>([&](auto uu) { b.sig[decltype(uu)::value] = J.sigma2; });
+      }
+    };
+    auto tab = [&](const Buf& b, auto uu) {
+      constexpr int u = decltype(uu)::value;
+      Tab T;
+      T.g = b.g[u];
+      T.sig = b.sig[u];
+      T.wsh = 0.0f;
+      T.wdh = 0.0f;
+      if constexpr (SHP != SHP_NONE) T.wsh = b.wsh[u];
+      if constexpr (SHP == SHP_GENERAL) T.wdh = b.wdh[u];
+      return T;
+    };
+    Buf bufA, bufB;
+    float2 col[BH];
+    load_chunk(std::integral_constant<int, 0>{}, bufA);
+    static_for<0, BH>([&](auto i) { col[i] = spec[cb * LSPEC + decltype(i)::value * SR + cc]; });
+    fft<BH, -1>(col);
+    if constexpr (MODE == MODE_W3D) {
+      if (col_on) {
+        float2* out = L.ring + (size_t)((m + TO) % BT) * L.spec_elems + colo;
+        static_for<0, BH>([&](auto i) { out[decltype(i)::value * COLS] = col[i]; });
+      }
+    }
+    // ---- 4a. per-block degrid fractions from the input DC and from the
+    // filtered DC (code_impl_C.h:64); lane 0 of each block computes both
+    if (col_on && cc == 0) {
+      const Tab T0 = tab(bufA, std::integral_constant<int, 0>{});
+      float gf = 0.0f, dcf;
+      if constexpr (MODE == MODE_W3D) {
+        float2 w[5];
+#pragma unroll
+        for (int q = 0; q < 5; q++) w[q] = make_float2(0.0f, 0.0f);
+        static_for<0, NQ>([&](auto qq) { w[decltype(qq)::value] = bufA.v[decltype(qq)::value][0]; });
+        w[NQ] = col[0];
+        gf = J.use_degrid ? J.degrid * w[BT / 2].x / grid0 : 0.0f;
+        dcf = apply3d_elem<BT>(J, w, T0, gf).x;
+      } else if constexpr (MODE == MODE_W2D) {
+        gf = J.use_degrid ? J.degrid * col[0].x / grid0 : 0.0f;
+        dcf = apply2d_s<SHP>(J, col[0], T0, gf).x;
+      } else if constexpr (MODE == MODE_KALMAN) {
+        KalState st = bufA.ks[0];
+        dcf = kalman_elem(J, col[0], T0, st).x;
+      } else {
+        dcf = col[0].x;
+      }
+      s_gf[cb] = gf;
+      s_gfs[cb] = J.use_degrid ? J.degrid * dcf / grid0 : 0.0f;
+    }
+    __syncthreads();
+    // ---- 4b. spectral pass down the column, chunk k+1 loading under chunk k
+    {
+      const float gf = s_gf[cb], gfs = s_gfs[cb];
+      static_for<0, NCH>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        if constexpr (k + 1 < NCH) load_chunk(std::integral_constant<int, k + 1>{}, pick<(k + 1) & 1>(bufA, bufB));
+        Buf& b = pick<k & 1>(bufA, bufB);
+        static_for<0, CH>([&](auto uu) {
+          constexpr int u = decltype(uu)::value, i = k * CH + u;
+          const Tab T = tab(b, uu);
+          float2 r;
+          if constexpr (MODE == MODE_W3D) {
+            float2 w[5];
+#pragma unroll
+            for (int q = 0; q < 5; q++) w[q] = make_float2(0.0f, 0.0f);
+            static_for<0, NQ>([&](auto qq) { w[decltype(qq)::value] = b.v[decltype(qq)::value][u]; });
+            w[NQ] = col[i];
+            r = apply3d_elem<BT>(J, w, T, gf);
+            if constexpr (SHP != SHP_NONE) r = sharpen_s<SHP>(J, r, T, gfs);
+          } else if constexpr (MODE == MODE_W2D) {
+            r = apply2d_s<SHP>(J, col[i], T, gf);
+            if constexpr (SHP != SHP_NONE) {
+              const float2 rs = sharpen_s<SHP>(J, r, T, gfs);
+              r = J.use_pattern ? rs : r;
+            }
+          } else if constexpr (MODE == MODE_KALMAN) {
+            KalState st = b.ks[u];
+            r = kalman_elem(J, col[i], T, st);
+            if (col_on) {
+              const size_t o = colo + (size_t)i * COLS;
+              J.kal_last[o] = st.last;
+              J.kal_covar[o] = st.covar;
+              J.kal_cproc[o] = st.cproc;
+            }
+            if constexpr (SHP != SHP_NONE) r = sharpen_s<SHP>(J, r, T, gfs);
+          } else {
+            r = sharpen_s<SHP>(J, col[i], T, gfs);
+          }
+          col[i] = r;
+        });
+        // chunk k's results are pinned here (an empty asm that clobbers
+        // memory): the compiler neither sinks the gains below the later
+        // chunks' loads nor hoists those loads above them, so at most two
+        // chunks of loads are in registers at once
+        static_for<0, CH>([&](auto uu) {
+          float2& cv = col[k * CH + decltype(uu)::value];
+          float a = cv.x, b = cv.y;
+          asm volatile("" : "+v"(a), "+v"(b) : : "memory");
+          cv.x = a;
+          cv.y = b;
+        });
+      });
+    }
+    // ---- 5. column inverse FFT, back to LDS for the row c2r
+    fft<BH, +1>(col);
+    if (col_on) {
+      static_for<0, BH>([&](auto i) { spec[cb * LSPEC + decltype(i)::value * SR + cc] = col[i]; });
+    }
+    __syncthreads();
+    // ---- 6. row c2r (Hermitian rows packed in pairs; DC and Nyquist imag dropped)
+    {
+      float2 z[BW];
+      const float2* src = spec + rb * LSPEC + 2 * rp * SR;
+      static_for<0, COLS>([&](auto kk) {
+        constexpr int k = decltype(kk)::value;
+        float2 a = src[k], b = src[SR + k];
+        if constexpr (k == 0 || 2 * k == BW) {
+          a.y = 0.0f;
+          b.y = 0.0f;
+        }
+        z[k] = make_float2(a.x - b.y, a.y + b.x);
+        if constexpr (k != 0 && 2 * k != BW) z[BW - k] = make_float2(a.x + b.y, b.x - a.y);
+      });
+      __syncthreads();  // spectral layout dead; synthesised rows next
+      fft<BW, +1>(z);
+      if (row_on) {
+        float* d0 = syn + rb * LSYN + 2 * rp * SYNW;
+        static_for<0, BW>([&](auto j) {
+          d0[decltype(j)::value] = z[j].x;
+          d0[SYNW + decltype(j)::value] = z[j].y;
+        });
+      }
+    }
+    __syncthreads();
+    // ---- the next frame's pixels load under the overlap-add (the last
+    // iteration reloads its own frame: an unconditional load keeps the old
+    // values from living through the iteration)
+    issue(J, L.src[min(fi + 1, i1 - 1)], ro, t);
+    // ---- 7. overlap-add inside the tile; seam partials to this frame's seam buffer
+    uint8_t* dst = L.dst[fi];
+    float* seam = J.seam + (size_t)fi * 4 * (size_t)J.seam_plane;
+    if (pix == 1) overlap_run<1, BW, BH, NT, LSYN, SYNW>(J, dst, seam, syn, by, bx0, nblk, RW, stepx, stepy, t);
+    else if (pix == 2) overlap_run<2, BW, BH, NT, LSYN, SYNW>(J, dst, seam, syn, by, bx0, nblk, RW, stepx, stepy, t);
+    else overlap_run<4, BW, BH, NT, LSYN, SYNW>(J, dst, seam, syn, by, bx0, nblk, RW, stepx, stepy, t);
+    __syncthreads();  // syn read; the next frame's gather writes the stage
+  }
+}
+
+// Seam pixels of every frame of a k_run launch: one workgroup per output row
+// and frame; rows inside a horizontal band finish the whole row, the others
+// only the columns of the vertical seams between tiles.
+template <int BW, int BH, int NT>
+__global__ void __launch_bounds__(256) k_seam_run(RunLaunch L_arg) {
+  const RunLaunch& L = kernarg_run(L_arg);
+  const PlaneJob& J = L.J;
+  constexpr int BPW = TShape<BW, BH, NT>::BPW;
+  const int yo = blockIdx.x;
+  const int fi = blockIdx.y;
+  const float* sm = J.seam + (size_t)fi * 4 * (size_t)J.seam_plane;
+  uint8_t* dst = L.dst[fi];
+  const int stepx = BW - J.ow, stepy = BH - J.oh;
+  int ry = yo;
+  if (J.interlaced) ry = (yo & 1) == 0 ? (yo >> 1) : (J.Hc >> 1) + ((J.Hc - 1 - yo) >> 1);
+  const int cy = ry + J.mirh, r = cy / stepy, i = cy - r * stepy;
+  if (i < J.oh && r >= 1) {
+    for (int x = threadIdx.x; x < J.Wc; x += 256) seam_pixel<BW, BH, NT>(J, sm, dst, yo, x);
+  } else {
+    const int nseam = (J.nox - 1) / BPW;  // tile boundaries at block columns BPW, 2*BPW, ...
+    for (int k = threadIdx.x; k < nseam * J.ow; k += 256) {
+      const int c = (k / J.ow + 1) * BPW, j = k - (k / J.ow) * J.ow;
+      const int x = c * stepx + j - J.mirw;
+      if (x >= 0 && x < J.Wc) seam_pixel<BW, BH, NT>(J, sm, dst, yo, x);
+    }
+  }
+}
+
+template <int BW, int BH>
+hipError_t launch_run_shape(int mode, int bt, int shp, const RunLaunch& L, hipStream_t s) {
+  constexpr int NT = 128;
+  if (L.nframes < 1 || L.nframes > RUN_MAX_FRAMES || L.fpw < 1) return hipErrorInvalidValue;
+  const int chunks = (L.nframes + L.fpw - 1) / L.fpw;
+  dim3 grid(L.ntiles * chunks), block(NT);
+#define NF3D_RUN(M, B, H) hipLaunchKernelGGL((k_run<BW, BH, NT, M, B, H>), grid, block, 0, s, L)
+#define NF3D_RUN_SHP(M, B)                                   \
+  switch (shp) {                                             \
+    case SHP_NONE: NF3D_RUN(M, B, SHP_NONE); break;          \
+    case SHP_SHARPEN: NF3D_RUN(M, B, SHP_SHARPEN); break;    \
+    case SHP_GENERAL: NF3D_RUN(M, B, SHP_GENERAL); break;    \
+    default: return hipErrorInvalidValue;                    \
+  }
+  switch (mode) {
+    case MODE_W2D: NF3D_RUN_SHP(MODE_W2D, 0); break;
+    case MODE_KALMAN: NF3D_RUN_SHP(MODE_KALMAN, 0); break;
+    case MODE_SHARPEN:
+      if (shp == SHP_SHARPEN) NF3D_RUN(MODE_SHARPEN, 0, SHP_SHARPEN);
+      else if (shp == SHP_GENERAL) NF3D_RUN(MODE_SHARPEN, 0, SHP_GENERAL);
+      else return hipErrorInvalidValue;
+      break;
+    case MODE_W3D:
+      switch (bt) {
+        case 2: NF3D_RUN_SHP(MODE_W3D, 2); break;
+        case 3: NF3D_RUN_SHP(MODE_W3D, 3); break;
+        case 4: NF3D_RUN_SHP(MODE_W3D, 4); break;
+        case 5: NF3D_RUN_SHP(MODE_W3D, 5); break;
+        default: return hipErrorInvalidValue;
+      }
+      break;
+    default: return hipErrorInvalidValue;
+  }
+#undef NF3D_RUN_SHP
+#undef NF3D_RUN
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  dim3 sgrid(L.J.Hc, L.nframes);
+  hipLaunchKernelGGL((k_seam_run<BW, BH, NT>), sgrid, dim3(256), 0, s, L);
+  return hipGetLastError();
+}
+
+}  // namespace nf3d

@@ -58,8 +58,8 @@ __device__ __forceinline__ int interior_to_out_row(const PlaneJob& J, int ry) {
 // Rounding and clamping of one output pixel (src/buffer.cpp:406: truncation
 // toward zero after adding 0.5 and planeBase, then [0, 2^bits-1]; float
 // samples clamp to [0,1]).
-__device__ __forceinline__ void store_px(const PlaneJob& J, int yo, int x, float v) {
-  uint8_t* row = J.dst + (long long)(yo + J.tc) * J.dst_pitch;
+__device__ __forceinline__ void store_px(const PlaneJob& J, uint8_t* dst, int yo, int x, float v) {
+  uint8_t* row = dst + (long long)(yo + J.tc) * J.dst_pitch;
   const int xx = x + J.lc;
   if (J.pix == 4) {
     float o = v < 0.0f ? 0.0f : v;
@@ -104,7 +104,7 @@ __device__ __forceinline__ float fast_div(float a, float b) { return a * __built
 __device__ __forceinline__ float sharpen_factor(const PlaneJob& J, float psd, float wsh) {
   float a = J.sharpen * wsh;
   float q = fast_div(psd * J.smax_n, (psd + J.smin_n) * (psd + J.smax_n));
-  return 1.0f + a * __builtin_sqrtf(q);
+  return 1.0f + a * __builtin_amdgcn_sqrtf(q);
 }
 __device__ __forceinline__ float dehalo_factor(const PlaneJob& J, float psd, float wdh) {
   float s = psd + J.ht2n;
@@ -375,8 +375,8 @@ __device__ __forceinline__ void gather_tile(const PlaneJob& J, float2* stage, co
 // (a zero weight drops a side exactly: exclusive columns have wa = 0, wb = 1).
 // Row classes are uniform across the workgroup.
 template <int PIX, int BW, int BH, int NT, int LSYN, int SYNW>
-__device__ __forceinline__ void overlap_tile(const PlaneJob& J, const float* syn, int by, int bx0, int nblk, int RW,
-                                             int stepx, int stepy, int t) {
+__device__ __forceinline__ void overlap_tile(const PlaneJob& J, uint8_t* dst, float* seam, const float* syn, int by,
+                                             int bx0, int nblk, int RW, int stepx, int stepy, int t) {
   using T = typename std::conditional<PIX == 1, uint8_t, typename std::conditional<PIX == 2, uint16_t, float>::type>::type;
   const float rounder = PIX == 4 ? 0.0f : 0.5f;
   const float fbase = (float)J.obase;
@@ -413,7 +413,7 @@ __device__ __forceinline__ void overlap_tile(const PlaneJob& J, const float* syn
       wb = 0.0f;
     }
     const size_t seam_slot_x = (size_t)(cls == 3 ? 0 : 1) * J.seam_plane;  // column seams
-    T* dcol = reinterpret_cast<T*>(J.dst + (long long)J.tc * J.dst_pitch) + x + J.lc;
+    T* dcol = reinterpret_cast<T*>(dst + (long long)J.tc * J.dst_pitch) + x + J.lc;
     for (int i = 0; i < BH; i++) {
       const int ry = cy0 + i - J.mirh;
       if (ry < 0 || ry >= J.Hc) continue;  // padding rows (uniform)
@@ -434,18 +434,18 @@ __device__ __forceinline__ void overlap_tile(const PlaneJob& J, const float* syn
             *d = (T)(q > J.maxval ? J.maxval : q);
           }
         } else {
-          J.seam[seam_slot_x + so] = h;  // slot 0 = left block's part, slot 1 = right's
+          seam[seam_slot_x + so] = h;  // slot 0 = left block's part, slot 1 = right's
         }
       } else {
         const bool ytop = i < J.oh;
         if (cls <= 1) {
           // row seam: slot 0 = upper block row's part, slot 1 = lower's
           const float w = ytop ? J.wsynyl[i] * J.norm : J.wsynyr[i - stepy] * J.norm;
-          J.seam[(size_t)(ytop ? 1 : 0) * J.seam_plane + so] = h * w;
+          seam[(size_t)(ytop ? 1 : 0) * J.seam_plane + so] = h * w;
         } else {
           // corner: A upper-left, B upper-right, C lower-left, D lower-right
           const int slot = ytop ? (cls == 3 ? 2 : 3) : (cls == 3 ? 0 : 1);
-          J.seam[(size_t)slot * J.seam_plane + so] = h;
+          seam[(size_t)slot * J.seam_plane + so] = h;
         }
       }
     }
@@ -663,27 +663,17 @@ __global__ void __launch_bounds__(NT) k_tile(TileLaunch L_arg) {
       __syncthreads();
     }
     // ---- 7. overlap-add inside the tile; seam partials to the seam buffer
-    if (pix == 1) overlap_tile<1, BW, BH, NT, LSYN, SYNW>(J, syn, by, bx0, nblk, RW, stepx, stepy, t);
-    else if (pix == 2) overlap_tile<2, BW, BH, NT, LSYN, SYNW>(J, syn, by, bx0, nblk, RW, stepx, stepy, t);
-    else overlap_tile<4, BW, BH, NT, LSYN, SYNW>(J, syn, by, bx0, nblk, RW, stepx, stepy, t);
+    if (pix == 1) overlap_tile<1, BW, BH, NT, LSYN, SYNW>(J, J.dst, J.seam, syn, by, bx0, nblk, RW, stepx, stepy, t);
+    else if (pix == 2) overlap_tile<2, BW, BH, NT, LSYN, SYNW>(J, J.dst, J.seam, syn, by, bx0, nblk, RW, stepx, stepy, t);
+    else overlap_tile<4, BW, BH, NT, LSYN, SYNW>(J, J.dst, J.seam, syn, by, bx0, nblk, RW, stepx, stepy, t);
   }
 }
 
-// Seam pixels: bands shared by two or four tiles, finished from the partial
-// sums in the reference's float order (src/buffer.cpp:445-477).
+// One seam pixel: a band shared by two or four tiles, finished from the
+// partial sums in the reference's float order (src/buffer.cpp:445-477).
 template <int BW, int BH, int NT>
-__global__ void __launch_bounds__(256) k_seam(TileLaunch L_arg) {
-  const TileLaunch& L = kernarg_launch(L_arg);
-  using S = TShape<BW, BH, NT>;
-  constexpr int BPW = S::BPW;
-  int g = blockIdx.y;
-  int jb = 0;
-  if (L.njobs > 1 && g >= L.job[1].seam_row_begin) jb = 1;
-  if (L.njobs > 2 && g >= L.job[2].seam_row_begin) jb = 2;
-  const PlaneJob& J = L.job[jb];
-  const int yo = g - J.seam_row_begin;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (yo >= J.Hc || x >= J.Wc) return;
+__device__ __forceinline__ void seam_pixel(const PlaneJob& J, const float* sm, uint8_t* dst, int yo, int x) {
+  constexpr int BPW = TShape<BW, BH, NT>::BPW;
   const int stepx = BW - J.ow, stepy = BH - J.oh;
   int ry = yo;
   if (J.interlaced) ry = (yo & 1) == 0 ? (yo >> 1) : (J.Hc >> 1) + ((J.Hc - 1 - yo) >> 1);
@@ -693,7 +683,6 @@ __global__ void __launch_bounds__(256) k_seam(TileLaunch L_arg) {
   const bool xseam = j < J.ow && c >= 1 && (c % BPW) == 0;
   if (!yband && !xseam) return;
   const size_t o = (size_t)yo * J.Wc + x;
-  const float* sm = J.seam;
   const size_t P = J.seam_plane;
   const float rounder = J.pix == 4 ? 0.0f : 0.5f;
   const float fbase = (float)J.obase;
@@ -707,7 +696,21 @@ __global__ void __launch_bounds__(256) k_seam(TileLaunch L_arg) {
   } else {
     v = (sm[o] + sm[P + o]) * J.norm + rounder + fbase;
   }
-  store_px(J, yo, x, v);
+  store_px(J, dst, yo, x, v);
+}
+
+template <int BW, int BH, int NT>
+__global__ void __launch_bounds__(256) k_seam(TileLaunch L_arg) {
+  const TileLaunch& L = kernarg_launch(L_arg);
+  int g = blockIdx.y;
+  int jb = 0;
+  if (L.njobs > 1 && g >= L.job[1].seam_row_begin) jb = 1;
+  if (L.njobs > 2 && g >= L.job[2].seam_row_begin) jb = 2;
+  const PlaneJob& J = L.job[jb];
+  const int yo = g - J.seam_row_begin;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (yo >= J.Hc || x >= J.Wc) return;
+  seam_pixel<BW, BH, NT>(J, J.seam, J.dst, yo, x);
 }
 
 template <int BW, int BH>
