@@ -97,6 +97,14 @@ def gpu_run(args):
             e.sync()
         torch.cuda.synchronize()
 
+    if args.copy_calibration:
+        # a plain device copy of known size, for scaling HBM traffic counters
+        nbytes = args.copy_calibration << 20
+        a = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     for _ in range(args.warmup):
         step()
     sync()
@@ -236,6 +244,8 @@ def main():
     ap.add_argument("--frames", type=int, default=300)
     ap.add_argument("--bt", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="run one device copy of MIB MiB first (a reference kernel for traffic counters)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-frames", type=int, default=32)
     ap.add_argument("--cpu-threads", type=int, default=min(os.cpu_count() or 1, 16))

@@ -16,7 +16,8 @@ import ctypes
 import os
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "libnf3d.so")
+# NF3D_LIBRARY selects another build of the library (e.g. a tuned variant)
+LIB_PATH = os.environ.get("NF3D_LIBRARY") or os.path.join(PKG, "libnf3d.so")
 SYNTH_PATH = os.path.join(PKG, "libnf3d_synth.so")
 
 

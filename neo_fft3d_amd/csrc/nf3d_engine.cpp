@@ -77,6 +77,8 @@ struct nf3d_engine {
   std::vector<float> wanx, wany, wsynxl, wsynxr, wsynyl, wsynyr, wsharpen, wdehalo, pwin, pattern2d, pattern3d;
   std::vector<float2> grid;
   bool pattern_set = false, pattern3d_init = false;
+  bool pshow_flat = false;  // pattern display has set the windows to 1
+  float* d_ones = nullptr;  // flat window
 
   hipStream_t stream = nullptr;
   float* d_tab = nullptr;  // every float table, one allocation
@@ -107,9 +109,11 @@ struct nf3d_engine {
   std::vector<hipEvent_t> ev_pool;
   std::vector<int> ev_frames;  // frames filtered by each timed launch
   size_t ev_used = 0;
-  // frame-loop launches (k_run): per-frame seam buffers
+  // frame-loop launches (k_run): per-frame seam buffers, two launches' worth
+  // (consecutive launches alternate, so a launch does not wait on the seam
+  // pass of the one before for its buffer)
   float* d_seam_run = nullptr;
-  static constexpr int kRunFrames = 16;  // frames per k_run launch
+  static constexpr int kRunFrames = nf3d::RUN_MAX_FRAMES;  // frames per k_run launch
 
   ~nf3d_engine() {
     hipFree(d_tab);
@@ -120,6 +124,7 @@ struct nf3d_engine {
     hipFree(d_kal_cproc);
     hipFree(d_seam);
     hipFree(d_seam_run);
+    hipFree(d_ones);
     hipFree(d_spec_tmp);
     hipFree(d_psums);
     hipFree(d_choice);
@@ -356,7 +361,6 @@ struct nf3d_engine {
       fail("crop leaves the plane outside the block grid");
     if (ep.px < 0 || ep.py < 0 || ep.px >= nox || ep.py >= noy) fail("px/py outside the block grid");
     if (ep.interlaced && (Hc & 1)) fail("interlaced requires an even plane height");
-    if (ep.pshow) fail("pshow is not supported by the GPU engine");
     if (!nf3d::block_shape_supported(bw, bh)) fail("block size bw x bh not compiled for the GPU engine");
     build_windows();
     norm = 1.0f / (float)(bw * bh);
@@ -601,8 +605,69 @@ struct nf3d_engine {
     pattern3d_init = true;
   }
 
+  // Pattern display, src/fft3d_engine.h:505-565: frame n's pattern block
+  // (searched unless px/py are given) reconstructed alone, with analysis and
+  // synthesis windows set to 1 -- permanently: calls after the first search
+  // with flat windows too -- and the plane centred for every non-RGB plane,
+  // luma included.
+  void run_pshow(int n, const Call& c, uint8_t* dst, long long dst_pitch) {
+    Src s0 = source(c, n);
+    if (!d_spec_tmp) {
+      d_spec_tmp = dev_alloc<float2>(spec_elems);
+      d_psums = dev_alloc<float>(nb);
+      d_choice = dev_alloc<int>(2);
+    }
+    if (!d_ones) {
+      std::vector<float> ones(std::max(ep.bw, ep.bh), 1.0f);
+      d_ones = dev_alloc<float>(ones.size());
+      check(hipMemcpy(d_ones, ones.data(), ones.size() * sizeof(float), hipMemcpyHostToDevice), "ones");
+    }
+    auto flat = [&](PlaneJob& J) {
+      J.wanx = J.wany = J.wsynxl = J.wsynxr = J.wsynyl = J.wsynyr = d_ones;
+    };
+    PlaneJob F = base_job();
+    if (pshow_flat) flat(F);
+    F.src = s0.p;
+    F.src_pitch = s0.pitch;
+    F.spec_out = d_spec_tmp;
+    launch(nf3d::MODE_FWD, 0, F, c.s);
+    PatternJob P{};
+    P.nox = nox;
+    P.noy = noy;
+    P.bh = ep.bh;
+    P.outwidth = outwidth;
+    P.spec = d_spec_tmp;
+    P.grid = d_grid;
+    P.pwin = d_pwin;
+    P.degrid = ep.degrid;
+    P.px = ep.px;
+    P.py = ep.py;
+    P.pattern2d = d_pattern2d;
+    P.sums = d_psums;
+    P.choice = d_choice;
+    check(nf3d::launch_pattern(P, c.s), "pattern kernels");
+    pshow_flat = true;
+    PlaneJob J = base_job();
+    flat(J);
+    const bool centre = fmt.color_family != 2 && !fmt.is_float;
+    J.base = centre ? (float)(1 << (fmt.bits_per_sample - 1)) : 0.0f;
+    J.obase = centre ? (1 << (fmt.bits_per_sample - 1)) : 0;
+    J.sharpen = 0.0f;
+    J.dehalo = 0.0f;
+    J.keep_choice = d_choice;
+    J.src = s0.p;
+    J.src_pitch = s0.pitch;
+    J.dst = dst;
+    J.dst_pitch = dst_pitch;
+    launch(nf3d::MODE_SHARPEN, 0, J, c.s, true);
+  }
+
   void run(int n, const Call& c, uint8_t* dst, long long dst_pitch) {
     check(hipSetDevice(device), "hipSetDevice");
+    if (ep.pfactor != 0 && ep.pshow) {
+      run_pshow(n, c, dst, dst_pitch);
+      return;
+    }
     if (ep.pfactor != 0 && !pattern_set) estimate_pattern(c);
     int btcur = ep.bt;
     if ((ep.bt / 2 > n) || (ep.bt - 1) / 2 > (frames - 1 - n)) btcur = 1;
@@ -709,10 +774,14 @@ struct nf3d_engine {
     }
     const int shp = shp_kind();
     const int a = std::max(first, lo), b = std::min(first + count - 1, hi);
-    // k_run keeps 32-bit row offsets; the sharpen-only mode without sharpen
-    // or dehalo is a plain round trip left to the per-frame path
-    const bool fits = (long long)(tc + Hc) * src_pitch < (1LL << 31);
-    if (a > b || !fits || (mode == nf3d::MODE_SHARPEN && shp == nf3d::SHP_NONE)) {
+    // k_run keeps 32-bit offsets (source rows, destination and seam
+    // elements); the sharpen-only mode without sharpen or dehalo is a plain
+    // round trip left to the per-frame path
+    const bool fits = (long long)(tc + Hc) * src_pitch < (1LL << 31) &&
+                      (long long)(tc + Hc) * dst_pitch < (1LL << 31) && 4LL * Hc * Wc < (1LL << 31) &&
+                      dst_pitch % pix == 0;
+    const bool pshow = ep.pfactor != 0 && ep.pshow;
+    if (a > b || !fits || pshow || (mode == nf3d::MODE_SHARPEN && shp == nf3d::SHP_NONE)) {
       for (int n = first; n < first + count; n++) run_one(n, src, src_pitch, dst, dst_pitch, s);
       return;
     }
@@ -735,13 +804,17 @@ struct nf3d_engine {
         launch(nf3d::MODE_FWD, 0, F, s);
       }
     }
-    if (!d_seam_run) d_seam_run = dev_alloc<float>((size_t)kRunFrames * 4 * Hc * Wc);
-    for (int m0 = a; m0 <= b; m0 += kRunFrames) {
-      const int nf = std::min(kRunFrames, b - m0 + 1);
+    // launches of equal length (within one frame)
+    const size_t seam_frame = (size_t)4 * Hc * Wc;
+    if (!d_seam_run) d_seam_run = dev_alloc<float>(2 * kRunFrames * seam_frame);
+    const int total = b - a + 1;
+    const int nl = (total + kRunFrames - 1) / kRunFrames;
+    for (int j = 0, m0 = a; j < nl; j++) {
+      const int nf = total / nl + (j >= nl - total % nl ? 1 : 0);
       RunLaunch R{};
       R.J = base_job();
       R.J.sigma2 = (float)(mode == nf3d::MODE_W3D ? ep.bt : 1) * ep.sigma * ep.sigma / norm;
-      R.J.seam = d_seam_run;
+      R.J.seam = d_seam_run + (size_t)(j & 1) * kRunFrames * seam_frame;
       R.J.src_pitch = src_pitch;
       R.J.dst_pitch = dst_pitch;
       for (int i = 0; i < nf; i++) {
@@ -756,7 +829,9 @@ struct nf3d_engine {
       R.ntiles = noy * ntx;
       auto ev = timer(true, nf, s);
       check(nf3d::launch_run(mode, ep.bt, shp, ep.bw, ep.bh, R, s), "frame-loop kernel");
+      check(nf3d::launch_seam_run(ep.bw, ep.bh, R, s), "seam kernel");
       if (ev.second) check(hipEventRecord(ev.second, s), "hipEventRecord");
+      m0 += nf;
     }
     if (mode == nf3d::MODE_W3D) {
       // slot (k % bt) now holds frame k for the last bt frames transformed

@@ -49,6 +49,7 @@ struct PlaneJob {
   const float* wdehalo;
   const float* pattern2d;
   const float* pattern3d;
+  const int* keep_choice;  // pattern display: keep only block (keep_choice[0], keep_choice[1]); null = off
   // spectral parameters (src/code_impl/code_impl.h:16-39)
   float sigma2;       // btcur * sigma^2 / norm
   float sigma2d;      // sigma^2 / norm (Kalman)
@@ -75,7 +76,7 @@ struct TileLaunch {
 
 // Frame-loop launch (k_run): one plane, up to RUN_MAX_FRAMES consecutive
 // frames, every tile of the plane.
-constexpr int RUN_MAX_FRAMES = 32;
+constexpr int RUN_MAX_FRAMES = 16;
 
 // Specialisation of the sharpen / dehalo factors in k_run's spectral pass.
 enum Shp : int {
@@ -113,8 +114,10 @@ struct PatternJob {
 // launch_tile runs k_tile and, for the filtering modes, k_seam.
 hipError_t launch_tile(int mode, int bt, int onchip, int bw, int bh, const TileLaunch& l, hipStream_t s);
 hipError_t launch_pattern(const PatternJob& job, hipStream_t s);
-// k_run over L.nframes frames, then k_seam_run; `shp` is a Shp value.
+// k_run over L.nframes frames; `shp` is a Shp value.  launch_seam_run
+// finishes the seams of L's frames (after its k_run, same stream).
 hipError_t launch_run(int mode, int bt, int shp, int bw, int bh, const RunLaunch& l, hipStream_t s);
+hipError_t launch_seam_run(int bw, int bh, const RunLaunch& l, hipStream_t s);
 bool block_shape_supported(int bw, int bh);
 int tile_blocks_per_wg(int bw, int bh);
 

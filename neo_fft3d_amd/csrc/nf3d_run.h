@@ -6,11 +6,12 @@
 // others between frames.  Per frame the phases are those of k_tile (gather,
 // row r2c, column FFT, spectral pass, column inverse, row c2r, overlap-add);
 // what changes is where the data waits are:
-//   - the pixels of the next frame are loaded into registers during the
-//     current frame's overlap-add, so the gather finds them resident;
-//   - the spectral pass runs down each column in chunks of rows, the ring
-//     spectra, Kalman state and table rows of chunk k+1 loaded while chunk k
-//     is computed; chunk 0 is loaded before the forward column FFT;
+//   - the pixels of the next frame are loaded into registers at the end of
+//     the current frame, in flight across the barrier;
+//   - the spectral pass runs down each column in chunks of rows: the ring
+//     spectra / Kalman state of chunk k+DR (DR up to 3) and the table rows of
+//     chunk k+1 are loaded while chunk k is computed; the first chunks are
+//     loaded before the forward column FFT;
 //   - the gains are specialised at compile time on the sharpen / dehalo case
 //     (SHP) and degrid is branch-free (a zero fraction when it is off).
 // Wiener 3D: the newest frame of each window (frame m + (BT-1)/2) is
@@ -85,7 +86,9 @@ template <int PIX, int BW, int BH, int NT, int SA, int LA>
 __device__ __forceinline__ void gather_window(const PlaneJob& J, float2* stage, const uint32_t (&raw)[2][BH],
                                               const float* winx, const float* winy, int by, int bx0, int nblk,
                                               int RW, int stepx, int t) {
-  const int ow = J.ow, oh = J.oh;
+  // launch fields as values: one load each, not one per row
+  const int ow = J.ow, oh = J.oh, nox = J.nox, noy = J.noy;
+  const float base = J.base;
   static_for<0, 2>([&](auto pp) {
     constexpr int p = decltype(pp)::value;
     if (p == 1 && RW <= NT) return;  // uniform
@@ -101,7 +104,7 @@ __device__ __forceinline__ void gather_window(const PlaneJob& J, float2* stage, 
       side = j < ow ? 1 : (j >= BW - ow ? 2 : 0);
       wx = winx[j];
       const int bx = bx0 + b;
-      xedge = (bx == 0 && side == 1) || (bx == J.nox - 1 && side == 2);
+      xedge = (bx == 0 && side == 1) || (bx == nox - 1 && side == 2);
     };
     float wxA, wxB;
     int sA, sB;
@@ -111,51 +114,74 @@ __device__ __forceinline__ void gather_window(const PlaneJob& J, float2* stage, 
     float* dA = reinterpret_cast<float*>(&stage[bA * LA + jA]);
     float* dB = reinterpret_cast<float*>(&stage[bB * LA + jB]);
     if (!cv) return;  // tail lanes: nothing to store
+    // per-lane factor of the rows outside the vertical bands: wx on a
+    // windowed column, else 1 ((1*p)*1 = p and (wx*p)*1 = wx*p exactly)
+    const float aA = sA != 0 ? wxA : 1.0f, aB = sB != 0 ? wxB : 1.0f;
+    const bool top_edge = by == 0, bot_edge = by == noy - 1;  // uniform
+    float wyc[8];  // window values of the current 8 rows, read ahead of their use
     static_for<0, BH>([&](auto rr) {
       constexpr int i = decltype(rr)::value;
+      if constexpr (i % 8 == 0) static_for<0, 8>([&](auto uu) { wyc[decltype(uu)::value] = winy[i + decltype(uu)::value]; });
       float v;
       if constexpr (PIX == 4) v = __uint_as_float(raw[p][i]);
       else v = (float)raw[p][i];
-      const float px = v - J.base;
-      const bool ytop = i < oh, ybot = i >= BH - oh;
-      const bool yw = ytop || ybot;
-      const bool yedge = (by == 0 && ytop) || (by == J.noy - 1 && ybot);
-      const float wy = winy[i];
-      auto windowed = [&](float wx, int side, bool xedge) {
-        float a, b;
-        if (side != 0 && yw) {
-          a = yedge ? (xedge ? wx * wy : wy) : wx;
-          b = yedge ? (xedge ? 1.0f : wx) : wy;
-        } else {
-          a = side != 0 ? wx : (yw ? wy : 1.0f);
-          b = 1.0f;
-        }
-        return (a * px) * b;
-      };
+      const float px = v - base;
+      constexpr bool ytop = i < (BH / 2), ybot = !ytop;  // which band, if any
+      const bool yw = ytop ? i < oh : i >= BH - oh;     // uniform
       constexpr int off = (i >> 1) * SA * 2 + (i & 1);
-      dA[off] = windowed(wxA, sA, eA);
-      dB[off] = windowed(wxB, sB, eB);
+      if (!yw) {
+        dA[off] = aA * px;
+        dB[off] = aB * px;
+      } else {
+        const float wy = wyc[i % 8];
+        const bool yedge = ytop ? top_edge : bot_edge;  // uniform
+        auto windowed = [&](float wx, int side, bool xedge) {
+          float a, b;
+          if (!yedge) {
+            a = side != 0 ? wx : wy;
+            b = side != 0 ? wy : 1.0f;
+          } else {
+            a = side != 0 ? (xedge ? wx * wy : wy) : wy;
+            b = side != 0 ? (xedge ? 1.0f : wx) : 1.0f;
+          }
+          return (a * px) * b;
+        };
+        dA[off] = windowed(wxA, sA, eA);
+        dB[off] = windowed(wxB, sB, eB);
+      }
     });
   });
 }
 
 // Phase 7 for k_run: overlap-add of the tile in the reference's float order
 // (src/buffer.cpp:381-560), as overlap_tile; lanes compute every row and only
-// the stores are predicated.
+// the stores are predicated.  The synthesis windows come from LDS: a global
+// load here would wait (vmcnt counts stores too) for every store issued
+// before it.
 template <int PIX, int BW, int BH, int NT, int LSYN, int SYNW>
-__device__ __forceinline__ void overlap_run(const PlaneJob& J, uint8_t* dst, float* seam, const float* syn, int by,
-                                            int bx0, int nblk, int RW, int stepx, int stepy, int t) {
+__device__ __forceinline__ void overlap_run(const PlaneJob& J, uint8_t* dst, float* seam, const float* syn,
+                                            const float* wsyn, int by, int bx0, int nblk, int RW, int stepx,
+                                            int stepy, int t) {
+  // wsyn: synthesis windows in LDS, [left x (BW) | right x (BW) | top y (BH) |
+  // bottom y (BH)]; one base with computed indices (selecting between table
+  // pointers would put the pointers in scratch memory)
+  constexpr int XL = 0, XR = BW, YL = 2 * BW, YR = 2 * BW + BH;
   using T = typename std::conditional<PIX == 1, uint8_t, typename std::conditional<PIX == 2, uint16_t, float>::type>::type;
   const float rounder = PIX == 4 ? 0.0f : 0.5f;
-  const float fbase = (float)J.obase;
+  // launch fields as values: one load each, not one per row
+  const float fbase = (float)J.obase, norm = J.norm;
+  const int ow = J.ow, oh = J.oh, mirw = J.mirw, mirh = J.mirh, Wc = J.Wc, Hc = J.Hc, maxval = J.maxval;
+  const int interlaced = J.interlaced, half = J.Hc >> 1;
+  const long long dpitch = J.dst_pitch;
   const int cy0 = by * stepy, cx0 = bx0 * stepx;
   const size_t P = (size_t)J.seam_plane;
+  uint8_t* dbase = dst + (long long)J.tc * dpitch + (long long)J.lc * sizeof(T);
   static_for<0, 2>([&](auto pp) {
     constexpr int p = decltype(pp)::value;
     if (p == 1 && RW <= NT) return;  // uniform
     const int xl = t + p * NT;
-    const int x = cx0 + xl - J.mirw;
-    const bool xin = xl < RW && x >= 0 && x < J.Wc;
+    const int x = cx0 + xl - mirw;
+    const bool xin = xl < RW && x >= 0 && x < Wc;
     // x class: 0 exclusive column, 1 band inside the tile, 2 band shared with
     // the left tile (this tile holds its right block), 3 band shared with the
     // right tile (this tile holds its left block)
@@ -165,12 +191,12 @@ __device__ __forceinline__ void overlap_run(const PlaneJob& J, uint8_t* dst, flo
       const int bl = xl / stepx, j = xl - bl * stepx;
       lb = bl * LSYN + j;
       la = lb;
-      if (j < J.ow) {
-        wb = J.wsynxl[j];
+      if (j < ow) {
+        wb = wsyn[XL + j];
         if (bl >= 1) {
           cls = 1;
           la = (bl - 1) * LSYN + j + stepx;
-          wa = J.wsynxr[j];
+          wa = wsyn[XR + j];
         } else {
           cls = 2;
         }
@@ -180,43 +206,60 @@ __device__ __forceinline__ void overlap_run(const PlaneJob& J, uint8_t* dst, flo
       cls = 3;
       la = (nblk - 1) * LSYN + stepx + jj;
       lb = la;
-      wa = J.wsynxr[jj];
+      wa = wsyn[XR + jj];
       wb = 0.0f;
     }
     const bool own = xin && cls <= 1;   // pixel finished in this tile (body rows)
     const bool shr = xin && cls >= 2;   // column seam (body rows)
-    const size_t slot_x = (size_t)(cls == 3 ? 0 : 1) * P;
-    T* dcol = reinterpret_cast<T*>(dst + (long long)J.tc * J.dst_pitch) + x + J.lc;
-#pragma unroll 2
-    for (int i = 0; i < BH; i++) {
-      const int ry = cy0 + i - J.mirh;
-      if (ry < 0 || ry >= J.Hc) continue;  // padding rows (uniform)
-      const float h = syn[la + i * SYNW] * wa + syn[lb + i * SYNW] * wb;
-      const int yo = interior_to_out_row(J, ry);
-      const size_t so = (size_t)yo * J.Wc + x;
-      if (i >= J.oh && i < stepy) {  // rows of this block row only (uniform)
-        const float v = h * J.norm + rounder + fbase;
-        T q;
-        if constexpr (PIX == 4) {
-          const float o = v < 0.0f ? 0.0f : v;
-          q = o > 1.0f ? 1.0f : o;
-        } else {
-          int qi = (int)v;
-          qi = qi < 0 ? 0 : qi;
-          q = (T)(qi > J.maxval ? J.maxval : qi);
+    // 32-bit element offsets from the uniform plane / seam bases (planes and
+    // seam buffers are < 2^31 elements: checked by the host)
+    const int slot_x = (cls == 3 ? 0 : 1) * (int)P;
+    const int xc = xin ? x : 0;
+    const int pitch_el = (int)(dpitch / (long long)sizeof(T));
+    T* dpl = reinterpret_cast<T*>(dbase);
+    // rows in chunks of 8: the chunk's LDS reads are issued together, ahead
+    // of the per-row (uniform) branches
+#pragma unroll 1
+    for (int i0 = 0; i0 < BH; i0 += 8) {
+      float h[8], w[8];
+      static_for<0, 8>([&](auto uu) {
+        constexpr int u = decltype(uu)::value;
+        const int i = i0 + u;
+        h[u] = syn[la + i * SYNW] * wa + syn[lb + i * SYNW] * wb;
+        w[u] = wsyn[i < oh ? YL + i : (i >= stepy ? YR + i - stepy : YL)];
+      });
+      static_for<0, 8>([&](auto uu) {
+        constexpr int u = decltype(uu)::value;
+        const int i = i0 + u;
+        const int ry = cy0 + i - mirh;
+        if (ry >= 0 && ry < Hc) {  // not a padding row (uniform)
+          // output row (interlaced fields stacked, second field bottom-up)
+          const int yo = !interlaced ? ry : (ry < half ? 2 * ry : Hc - 1 - 2 * (ry - half));
+          const int so = yo * Wc + xc;
+          if (i >= oh && i < stepy) {  // rows of this block row only (uniform)
+            const float v = h[u] * norm + rounder + fbase;
+            T q;
+            if constexpr (PIX == 4) {
+              const float o = v < 0.0f ? 0.0f : v;
+              q = o > 1.0f ? 1.0f : o;
+            } else {
+              int qi = (int)v;
+              qi = qi < 0 ? 0 : qi;
+              q = (T)(qi > maxval ? maxval : qi);
+            }
+            if (own) dpl[(unsigned)(yo * pitch_el + xc)] = q;
+            if (shr) seam[(unsigned)(slot_x + so)] = h[u];  // slot 0 = left block's part, slot 1 = right's
+          } else {
+            const bool ytop = i < oh;
+            // row seam: slot 0 = upper block row's part, slot 1 = lower's
+            // (times its window); corner: A upper-left, B upper-right, C
+            // lower-left, D lower-right
+            const int slot = cls <= 1 ? (ytop ? 1 : 0) : (ytop ? (cls == 3 ? 2 : 3) : (cls == 3 ? 0 : 1));
+            const float val = cls <= 1 ? h[u] * (w[u] * norm) : h[u];
+            if (xin) seam[(unsigned)(slot * (int)P + so)] = val;
+          }
         }
-        if (own) *reinterpret_cast<T*>(reinterpret_cast<uint8_t*>(dcol) + (long long)yo * J.dst_pitch) = q;
-        if (shr) seam[slot_x + so] = h;  // slot 0 = left block's part, slot 1 = right's
-      } else {
-        const bool ytop = i < J.oh;
-        // row seam: slot 0 = upper block row's part, slot 1 = lower's (times
-        // its window); corner: A upper-left, B upper-right, C lower-left,
-        // D lower-right
-        const float w = ytop ? J.wsynyl[i] * J.norm : J.wsynyr[i - stepy] * J.norm;
-        const int slot = cls <= 1 ? (ytop ? 1 : 0) : (ytop ? (cls == 3 ? 2 : 3) : (cls == 3 ? 0 : 1));
-        const float val = cls <= 1 ? h * w : h;
-        if (xin) seam[(size_t)slot * P + so] = val;
-      }
+      });
     }
   });
 }
@@ -226,7 +269,7 @@ __device__ __forceinline__ void overlap_run(const PlaneJob& J, uint8_t* dst, flo
 // for SHP_GENERAL the sharpen factor is exactly 1 when sharpen == 0, so the
 // product equals the reference's dehalo-only factor.
 template <int SHP>
-__device__ __forceinline__ float shp_factor(const PlaneJob& J, float psd, float wsh, float wdh) {
+__device__ __forceinline__ float shp_factor(const SpecParams& J, float psd, float wsh, float wdh) {
   if constexpr (SHP == SHP_SHARPEN) return sharpen_factor(J, psd, wsh);
   else return sharpen_factor(J, psd, wsh) * dehalo_factor(J, psd, wdh);
 }
@@ -234,7 +277,7 @@ __device__ __forceinline__ float shp_factor(const PlaneJob& J, float psd, float 
 // Sharpen pass of one element (src/code_impl/Sharpen.cpp:13-133); degrid
 // fraction zero when degrid is off.
 template <int SHP>
-__device__ __forceinline__ float2 sharpen_s(const PlaneJob& J, float2 x, const Tab& T, float gf) {
+__device__ __forceinline__ float2 sharpen_s(const SpecParams& J, float2 x, const Tab& T, float gf) {
   const float g0 = gf * T.g.x, g1 = gf * T.g.y;
   const float cr = x.x - g0, ci = x.y - g1;
   const float psd = cr * cr + ci * ci + 1.0e-15f;
@@ -245,7 +288,7 @@ __device__ __forceinline__ float2 sharpen_s(const PlaneJob& J, float2 x, const T
 // Apply2D (src/code_impl/Apply.cpp:13-65) with the sharpen factor folded in
 // when no pattern is used.
 template <int SHP>
-__device__ __forceinline__ float2 apply2d_s(const PlaneJob& J, float2 x, const Tab& T, float gf) {
+__device__ __forceinline__ float2 apply2d_s(const SpecParams& J, float2 x, const Tab& T, float gf) {
   const float g0 = gf * T.g.x, g1 = gf * T.g.y;
   const float cr = x.x - g0, ci = x.y - g1;
   const float psd = cr * cr + ci * ci + 1e-15f;
@@ -257,12 +300,16 @@ __device__ __forceinline__ float2 apply2d_s(const PlaneJob& J, float2 x, const T
   return make_float2(cr * f + g0, ci * f + g1);
 }
 
-// One chunk of CH rows of a lane's column: ring spectra, Kalman state and
-// table values, loaded together.
-template <int MODE, int NQ, int SHP, int CH>
-struct ChunkBuf {
+// One chunk of CH rows of a lane's column: the ring spectra / Kalman state
+// (HBM, loaded several chunks ahead) and the table values (cache-resident,
+// one chunk ahead).
+template <int MODE, int NQ, int CH>
+struct RingBuf {
   float2 v[NQ > 0 ? NQ : 1][CH];
   KalState ks[MODE == MODE_KALMAN ? CH : 1];
+};
+template <int SHP, int CH>
+struct TabBuf {
   float2 g[CH];
   float sig[CH];
   float wsh[SHP != SHP_NONE ? CH : 1];
@@ -286,12 +333,16 @@ __global__ void __launch_bounds__(NT, 2) k_run(RunLaunch L_arg) {
   constexpr int NQ = MODE == MODE_W3D ? BT - 1 : 0;  // ring frames per element
   constexpr int CH = (MODE == MODE_W3D) ? 4 : 2;  // rows per chunk (two chunks of loads in flight)
   constexpr int NCH = BH / CH;
-  static_assert(BH % CH == 0 && BH % 4 == 0 && BW % 4 == 0, "block shape");
+  static_assert(BH % CH == 0 && BH % 8 == 0 && BW % 4 == 0, "block shape");
   static_assert(MODE != MODE_FWD, "k_run filters");
-  using Buf = ChunkBuf<MODE, NQ, SHP, CH>;
+  // chunks of ring rows in flight ahead of the one being computed
+  constexpr int DR = MODE == MODE_W3D ? (NQ <= 2 ? 3 : NQ == 3 ? 2 : 1) : 1;
+  using RBuf = RingBuf<MODE, NQ, CH>;
+  using TBuf = TabBuf<SHP, CH>;
   __shared__ __align__(16) unsigned char smem[S::BYTES];
   __shared__ float s_gf[BPW], s_gfs[BPW];
   __shared__ float winx[BW], winy[BH];
+  __shared__ float wsyn[2 * BW + 2 * BH];  // synthesis windows: left, right, top, bottom bands
   __shared__ int rowoff[BH];  // byte offset of the source row of each tile row (< 2^31: checked by the host)
   float2* spec = reinterpret_cast<float2*>(smem);
   float2* stage = reinterpret_cast<float2*>(smem);
@@ -316,6 +367,14 @@ __global__ void __launch_bounds__(NT, 2) k_run(RunLaunch L_arg) {
     for (int i = t; i < BH; i += NT) {
       winy[i] = J.wany[i];
       rowoff[i] = (int)((long long)(cover_to_src_row(J, by * stepy + i) + J.tc) * J.src_pitch);
+    }
+    for (int i = t; i < J.ow; i += NT) {
+      wsyn[i] = J.wsynxl[i];
+      wsyn[BW + i] = J.wsynxr[i];
+    }
+    for (int i = t; i < J.oh; i += NT) {
+      wsyn[2 * BW + i] = J.wsynyl[i];
+      wsyn[2 * BW + BH + i] = J.wsynyr[i];
     }
   }
   __syncthreads();
@@ -358,6 +417,7 @@ __global__ void __launch_bounds__(NT, 2) k_run(RunLaunch L_arg) {
     const size_t colo = tile_spec0 + (size_t)cb * (BH * COLS) + cc;  // element (0, cc) of this lane's column
     const float* pat = J.use_pattern ? (MODE == MODE_W3D ? J.pattern3d : J.pattern2d) : nullptr;
     const float grid0 = J.grid[0].x;
+    const SpecParams SP(J);
 
     // ---- 1. window the loaded pixels into per-block row pairs
     if (pix == 1) gather_window<1, BW, BH, NT, SA, LA>(J, stage, raw, wx, wy, by, bx0, nblk, RW, stepx, t);
@@ -389,14 +449,20 @@ __global__ void __launch_bounds__(NT, 2) k_run(RunLaunch L_arg) {
       constexpr int q = decltype(qq)::value;
       rq[q] = L.ring + (size_t)((m + FROM + q) % BT) * L.spec_elems;
     });
-    auto load_chunk = [&](auto kc, Buf& b) {
+    auto load_ring = [&](auto kc, RBuf& b) {
       constexpr int k = decltype(kc)::value;
       static_for<0, CH>([&](auto uu) {
         constexpr int u = decltype(uu)::value, i = k * CH + u;
         const size_t o = colo + (size_t)i * COLS;
-        const int kk = i * COLS + cc;
         static_for<0, NQ>([&](auto qq) { b.v[decltype(qq)::value][u] = rq[decltype(qq)::value][o]; });
         if constexpr (MODE == MODE_KALMAN) b.ks[u] = KalState{J.kal_last[o], J.kal_covar[o], J.kal_cproc[o]};
+      });
+    };
+    auto load_tab = [&](auto kc, TBuf& b) {
+      constexpr int k = decltype(kc)::value;
+      static_for<0, CH>([&](auto uu) {
+        constexpr int u = decltype(uu)::value, i = k * CH + u;
+        const int kk = i * COLS + cc;
         b.g[u] = J.grid[kk];
         if constexpr (SHP != SHP_NONE) b.wsh[u] = J.wsharpen[kk];
         if constexpr (SHP == SHP_GENERAL) b.wdh[u] = J.wdehalo[kk];
@@ -407,7 +473,7 @@ __global__ void __launch_bounds__(NT, 2) k_run(RunLaunch L_arg) {
         static_for<0, CH>([&](auto uu) { b.sig[decltype(uu)::value] = J.sigma2; });
       }
     };
-    auto tab = [&](const Buf& b, auto uu) {
+    auto tab = [&](const TBuf& b, auto uu) {
       constexpr int u = decltype(uu)::value;
       Tab T;
       T.g = b.g[u];
@@ -418,9 +484,11 @@ __global__ void __launch_bounds__(NT, 2) k_run(RunLaunch L_arg) {
       if constexpr (SHP == SHP_GENERAL) T.wdh = b.wdh[u];
       return T;
     };
-    Buf bufA, bufB;
+    RBuf rbuf[DR + 1];
+    TBuf tbuf[2];
     float2 col[BH];
-    load_chunk(std::integral_constant<int, 0>{}, bufA);
+    static_for<0, (DR < NCH ? DR : NCH)>([&](auto kc) { load_ring(kc, rbuf[decltype(kc)::value]); });
+    load_tab(std::integral_constant<int, 0>{}, tbuf[0]);
     static_for<0, BH>([&](auto i) { col[i] = spec[cb * LSPEC + decltype(i)::value * SR + cc]; });
     fft<BH, -1>(col);
     if constexpr (MODE == MODE_W3D) {
@@ -432,22 +500,22 @@ __global__ void __launch_bounds__(NT, 2) k_run(RunLaunch L_arg) {
     // ---- 4a. per-block degrid fractions from the input DC and from the
     // filtered DC (code_impl_C.h:64); lane 0 of each block computes both
     if (col_on && cc == 0) {
-      const Tab T0 = tab(bufA, std::integral_constant<int, 0>{});
+      const Tab T0 = tab(tbuf[0], std::integral_constant<int, 0>{});
       float gf = 0.0f, dcf;
       if constexpr (MODE == MODE_W3D) {
         float2 w[5];
 #pragma unroll
         for (int q = 0; q < 5; q++) w[q] = make_float2(0.0f, 0.0f);
-        static_for<0, NQ>([&](auto qq) { w[decltype(qq)::value] = bufA.v[decltype(qq)::value][0]; });
+        static_for<0, NQ>([&](auto qq) { w[decltype(qq)::value] = rbuf[0].v[decltype(qq)::value][0]; });
         w[NQ] = col[0];
         gf = J.use_degrid ? J.degrid * w[BT / 2].x / grid0 : 0.0f;
-        dcf = apply3d_elem<BT>(J, w, T0, gf).x;
+        dcf = apply3d_elem<BT>(SP, w, T0, gf).x;
       } else if constexpr (MODE == MODE_W2D) {
         gf = J.use_degrid ? J.degrid * col[0].x / grid0 : 0.0f;
-        dcf = apply2d_s<SHP>(J, col[0], T0, gf).x;
+        dcf = apply2d_s<SHP>(SP, col[0], T0, gf).x;
       } else if constexpr (MODE == MODE_KALMAN) {
-        KalState st = bufA.ks[0];
-        dcf = kalman_elem(J, col[0], T0, st).x;
+        KalState st = rbuf[0].ks[0];
+        dcf = kalman_elem(SP, col[0], T0, st).x;
       } else {
         dcf = col[0].x;
       }
@@ -460,11 +528,13 @@ __global__ void __launch_bounds__(NT, 2) k_run(RunLaunch L_arg) {
       const float gf = s_gf[cb], gfs = s_gfs[cb];
       static_for<0, NCH>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
-        if constexpr (k + 1 < NCH) load_chunk(std::integral_constant<int, k + 1>{}, pick<(k + 1) & 1>(bufA, bufB));
-        Buf& b = pick<k & 1>(bufA, bufB);
+        if constexpr (k + DR < NCH) load_ring(std::integral_constant<int, k + DR>{}, rbuf[(k + DR) % (DR + 1)]);
+        if constexpr (k + 1 < NCH) load_tab(std::integral_constant<int, k + 1>{}, tbuf[(k + 1) & 1]);
+        const RBuf& b = rbuf[k % (DR + 1)];
+        const TBuf& tb = tbuf[k & 1];
         static_for<0, CH>([&](auto uu) {
           constexpr int u = decltype(uu)::value, i = k * CH + u;
-          const Tab T = tab(b, uu);
+          const Tab T = tab(tb, uu);
           float2 r;
           if constexpr (MODE == MODE_W3D) {
             float2 w[5];
@@ -472,26 +542,26 @@ __global__ void __launch_bounds__(NT, 2) k_run(RunLaunch L_arg) {
             for (int q = 0; q < 5; q++) w[q] = make_float2(0.0f, 0.0f);
             static_for<0, NQ>([&](auto qq) { w[decltype(qq)::value] = b.v[decltype(qq)::value][u]; });
             w[NQ] = col[i];
-            r = apply3d_elem<BT>(J, w, T, gf);
-            if constexpr (SHP != SHP_NONE) r = sharpen_s<SHP>(J, r, T, gfs);
+            r = apply3d_elem<BT>(SP, w, T, gf);
+            if constexpr (SHP != SHP_NONE) r = sharpen_s<SHP>(SP, r, T, gfs);
           } else if constexpr (MODE == MODE_W2D) {
-            r = apply2d_s<SHP>(J, col[i], T, gf);
+            r = apply2d_s<SHP>(SP, col[i], T, gf);
             if constexpr (SHP != SHP_NONE) {
-              const float2 rs = sharpen_s<SHP>(J, r, T, gfs);
-              r = J.use_pattern ? rs : r;
+              const float2 rs = sharpen_s<SHP>(SP, r, T, gfs);
+              r = SP.use_pattern ? rs : r;
             }
           } else if constexpr (MODE == MODE_KALMAN) {
             KalState st = b.ks[u];
-            r = kalman_elem(J, col[i], T, st);
+            r = kalman_elem(SP, col[i], T, st);
             if (col_on) {
               const size_t o = colo + (size_t)i * COLS;
               J.kal_last[o] = st.last;
               J.kal_covar[o] = st.covar;
               J.kal_cproc[o] = st.cproc;
             }
-            if constexpr (SHP != SHP_NONE) r = sharpen_s<SHP>(J, r, T, gfs);
+            if constexpr (SHP != SHP_NONE) r = sharpen_s<SHP>(SP, r, T, gfs);
           } else {
-            r = sharpen_s<SHP>(J, col[i], T, gfs);
+            r = sharpen_s<SHP>(SP, col[i], T, gfs);
           }
           col[i] = r;
         });
@@ -539,23 +609,28 @@ __global__ void __launch_bounds__(NT, 2) k_run(RunLaunch L_arg) {
       }
     }
     __syncthreads();
-    // ---- the next frame's pixels load under the overlap-add (the last
-    // iteration reloads its own frame: an unconditional load keeps the old
-    // values from living through the iteration)
-    issue(J, L.src[min(fi + 1, i1 - 1)], ro, t);
     // ---- 7. overlap-add inside the tile; seam partials to this frame's seam buffer
     uint8_t* dst = L.dst[fi];
     float* seam = J.seam + (size_t)fi * 4 * (size_t)J.seam_plane;
-    if (pix == 1) overlap_run<1, BW, BH, NT, LSYN, SYNW>(J, dst, seam, syn, by, bx0, nblk, RW, stepx, stepy, t);
-    else if (pix == 2) overlap_run<2, BW, BH, NT, LSYN, SYNW>(J, dst, seam, syn, by, bx0, nblk, RW, stepx, stepy, t);
-    else overlap_run<4, BW, BH, NT, LSYN, SYNW>(J, dst, seam, syn, by, bx0, nblk, RW, stepx, stepy, t);
+    const float* ws = wsyn + zo;
+    if (pix == 1) overlap_run<1, BW, BH, NT, LSYN, SYNW>(J, dst, seam, syn, ws, by, bx0, nblk, RW, stepx, stepy, t);
+    else if (pix == 2) overlap_run<2, BW, BH, NT, LSYN, SYNW>(J, dst, seam, syn, ws, by, bx0, nblk, RW, stepx, stepy, t);
+    else overlap_run<4, BW, BH, NT, LSYN, SYNW>(J, dst, seam, syn, ws, by, bx0, nblk, RW, stepx, stepy, t);
+    // ---- the next frame's pixels, in flight across the barrier (issued
+    // after the overlap-add's stores, which therefore need no slot or wait
+    // behind them; the last iteration reloads its own frame: an unconditional
+    // load keeps the old values from living through the iteration)
+    issue(J, L.src[min(fi + 1, i1 - 1)], ro, t);
     __syncthreads();  // syn read; the next frame's gather writes the stage
   }
 }
 
-// Seam pixels of every frame of a k_run launch: one workgroup per output row
-// and frame; rows inside a horizontal band finish the whole row, the others
-// only the columns of the vertical seams between tiles.
+// Seam pixels of every frame of a k_run launch, finished from the partial
+// sums in the reference's float order (src/buffer.cpp:445-477); one workgroup
+// per output row and frame.  Rows inside a horizontal band finish the whole
+// row: both band partials are loaded unconditionally (8 pixels per lane in
+// flight), the corner partials where a vertical seam crosses.  Other rows
+// finish only the columns of the vertical seams between tiles.
 template <int BW, int BH, int NT>
 __global__ void __launch_bounds__(256) k_seam_run(RunLaunch L_arg) {
   const RunLaunch& L = kernarg_run(L_arg);
@@ -569,14 +644,45 @@ __global__ void __launch_bounds__(256) k_seam_run(RunLaunch L_arg) {
   int ry = yo;
   if (J.interlaced) ry = (yo & 1) == 0 ? (yo >> 1) : (J.Hc >> 1) + ((J.Hc - 1 - yo) >> 1);
   const int cy = ry + J.mirh, r = cy / stepy, i = cy - r * stepy;
+  const float rounder = J.pix == 4 ? 0.0f : 0.5f;
+  const float fbase = (float)J.obase;
+  const size_t P = (size_t)J.seam_plane;
+  const float* row = sm + (size_t)yo * J.Wc;
   if (i < J.oh && r >= 1) {
-    for (int x = threadIdx.x; x < J.Wc; x += 256) seam_pixel<BW, BH, NT>(J, sm, dst, yo, x);
+    const float wyr = J.wsynyr[i] * J.norm, wyl = J.wsynyl[i] * J.norm;
+    constexpr int U = 8;
+    for (int x0 = 0; x0 < J.Wc; x0 += 256 * U) {
+      float up[U], lo[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int x = min(x0 + u * 256 + (int)threadIdx.x, J.Wc - 1);
+        up[u] = row[x];
+        lo[u] = row[P + x];
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int x = x0 + u * 256 + (int)threadIdx.x;
+        if (x >= J.Wc) break;
+        const int cx = x + J.mirw, c = cx / stepx, j = cx - c * stepx;
+        float v;
+        if (j < J.ow && c >= 1 && (c % BPW) == 0) {
+          const float A = up[u], B = lo[u], C = row[2 * P + x], D = row[3 * P + x];
+          v = (A + B) * wyr + (C + D) * wyl + rounder + fbase;
+        } else {
+          v = up[u] + lo[u] + rounder + fbase;
+        }
+        store_px(J, dst, yo, x, v);
+      }
+    }
   } else {
     const int nseam = (J.nox - 1) / BPW;  // tile boundaries at block columns BPW, 2*BPW, ...
     for (int k = threadIdx.x; k < nseam * J.ow; k += 256) {
       const int c = (k / J.ow + 1) * BPW, j = k - (k / J.ow) * J.ow;
       const int x = c * stepx + j - J.mirw;
-      if (x >= 0 && x < J.Wc) seam_pixel<BW, BH, NT>(J, sm, dst, yo, x);
+      if (x >= 0 && x < J.Wc) {
+        const float v = (row[x] + row[P + x]) * J.norm + rounder + fbase;
+        store_px(J, dst, yo, x, v);
+      }
     }
   }
 }
@@ -616,10 +722,14 @@ hipError_t launch_run_shape(int mode, int bt, int shp, const RunLaunch& L, hipSt
   }
 #undef NF3D_RUN_SHP
 #undef NF3D_RUN
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  return hipGetLastError();
+}
+
+template <int BW, int BH>
+hipError_t launch_seam_run_shape(const RunLaunch& L, hipStream_t s) {
+  if (L.nframes < 1 || L.nframes > RUN_MAX_FRAMES) return hipErrorInvalidValue;
   dim3 sgrid(L.J.Hc, L.nframes);
-  hipLaunchKernelGGL((k_seam_run<BW, BH, NT>), sgrid, dim3(256), 0, s, L);
+  hipLaunchKernelGGL((k_seam_run<BW, BH, 128>), sgrid, dim3(256), 0, s, L);
   return hipGetLastError();
 }
 

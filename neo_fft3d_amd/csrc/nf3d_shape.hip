@@ -13,6 +13,8 @@
 #define NF3D_NAME(a, b) NF3D_NAME2(a, b)
 #define NF3D_RNAME2(a, b) launch_run_##a##x##b
 #define NF3D_RNAME(a, b) NF3D_RNAME2(a, b)
+#define NF3D_SNAME2(a, b) launch_seam_run_##a##x##b
+#define NF3D_SNAME(a, b) NF3D_SNAME2(a, b)
 
 namespace nf3d {
 #if defined(NF3D_PART_TILE)
@@ -22,6 +24,9 @@ hipError_t NF3D_NAME(NF3D_BW, NF3D_BH)(int mode, int bt, int onchip, const TileL
 #elif defined(NF3D_PART_RUN)
 hipError_t NF3D_RNAME(NF3D_BW, NF3D_BH)(int mode, int bt, int shp, const RunLaunch& l, hipStream_t s) {
   return launch_run_shape<NF3D_BW, NF3D_BH>(mode, bt, shp, l, s);
+}
+hipError_t NF3D_SNAME(NF3D_BW, NF3D_BH)(const RunLaunch& l, hipStream_t s) {
+  return launch_seam_run_shape<NF3D_BW, NF3D_BH>(l, s);
 }
 #else
 #error "compile with -DNF3D_PART_TILE or -DNF3D_PART_RUN"

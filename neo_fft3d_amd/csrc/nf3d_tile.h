@@ -75,6 +75,16 @@ __device__ __forceinline__ void store_px(const PlaneJob& J, uint8_t* dst, int yo
 }
 
 // ---------------------------------------------------------------- spectral
+// The scalar parameters of the spectral pass, as values (k_run copies them
+// out of the launch record once per frame).
+struct SpecParams {
+  int use_degrid, use_pattern;
+  float lowlimit, sharpen, smin_n, smax_n, dehalo, ht2n, sigma2d, kratio2;
+  __device__ explicit SpecParams(const PlaneJob& J)
+      : use_degrid(J.use_degrid), use_pattern(J.use_pattern), lowlimit(J.lowlimit), sharpen(J.sharpen),
+        smin_n(J.smin_n), smax_n(J.smax_n), dehalo(J.dehalo), ht2n(J.ht2n), sigma2d(J.sigma2d), kratio2(J.kratio2) {}
+};
+
 // Per-frequency table values of one element, loaded together with the
 // spectra so a batch of elements waits on memory once.
 struct Tab {
@@ -101,12 +111,14 @@ __device__ __forceinline__ Tab load_tab(const PlaneJob& J, int k, bool sharp) {
 __device__ __forceinline__ float fast_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
 // Factors of src/code_impl/Sharpen.cpp:102-105.
-__device__ __forceinline__ float sharpen_factor(const PlaneJob& J, float psd, float wsh) {
+template <class PJ>
+__device__ __forceinline__ float sharpen_factor(const PJ& J, float psd, float wsh) {
   float a = J.sharpen * wsh;
   float q = fast_div(psd * J.smax_n, (psd + J.smin_n) * (psd + J.smax_n));
   return 1.0f + a * __builtin_amdgcn_sqrtf(q);
 }
-__device__ __forceinline__ float dehalo_factor(const PlaneJob& J, float psd, float wdh) {
+template <class PJ>
+__device__ __forceinline__ float dehalo_factor(const PJ& J, float psd, float wdh) {
   float s = psd + J.ht2n;
   return fast_div(s, s + J.dehalo * wdh * psd);
 }
@@ -157,8 +169,8 @@ __device__ __forceinline__ float2 sharpen_elem(const PlaneJob& J, float2 x, cons
 
 // Apply3D2..3D5 (src/code_impl/Apply.cpp:67-235), same float evaluation
 // order; v[0..BT-1] are the frames oldest first.
-template <int BT>
-__device__ __forceinline__ float2 apply3d_elem(const PlaneJob& J, const float2 (&v)[5], const Tab& T, float gf) {
+template <int BT, class PJ>
+__device__ __forceinline__ float2 apply3d_elem(const PJ& J, const float2 (&v)[5], const Tab& T, float gf) {
   const float sig = T.sig;
   const float ll = J.lowlimit;
   float g0 = 0.0f, g1 = 0.0f;
@@ -245,7 +257,8 @@ struct KalState {
   float2 last, covar, cproc;
 };
 
-__device__ __forceinline__ float2 kalman_elem(const PlaneJob& J, float2 cur, const Tab& T, KalState& st) {
+template <class PJ>
+__device__ __forceinline__ float2 kalman_elem(const PJ& J, float2 cur, const Tab& T, KalState& st) {
   float sigma = J.use_pattern ? (T.sig < 1e-15f ? 1e-15f : T.sig) : J.sigma2d;
   const float2 prev = st.last;
   const float dr = cur.x - prev.x, di = cur.y - prev.y;
@@ -626,6 +639,10 @@ __global__ void __launch_bounds__(NT) k_tile(TileLaunch L_arg) {
             if (do_sharp) r = sharpen_elem(J, r, tb[u], gfs);
           } else {
             r = do_sharp ? sharpen_elem(J, v[u][0], tb[u], gfs) : v[u][0];
+            // pattern display (src/helper.cpp:134-169): every block but the
+            // pattern block is zeroed
+            if (J.keep_choice && by * J.nox + bx0 + cb != J.keep_choice[1] * J.nox + J.keep_choice[0])
+              r = make_float2(0.0f, 0.0f);
           }
           col[i] = r;
         });

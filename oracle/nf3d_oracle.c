@@ -193,6 +193,7 @@ typedef struct {
   float *wsharpen, *wdehalo, *pwin;   /* [bh*outwidth] */
   double *pattern2d, *pattern3d;      /* [bh*outwidth] */
   int pattern_set, pattern3d_init;
+  int pshow_flat;                      /* pshow has set the windows to 1 */
   cplx* grid;                          /* [bh*outwidth] */
   float sigmaSquaredNoiseNormed2D, sigmaSquaredSharpenMinNormed, sigmaSquaredSharpenMaxNormed, ht2n;
   /* Kalman state */
@@ -684,7 +685,7 @@ static void engine_frame(oclip* c, int pl, int n, uint8_t* dst, int dstride) {
   int pattern = e->ep.pfactor != 0;
   for (int k = 0; k < n - 2 && k < e->frames; k++)
     if (e->spec[k] && k != e->ep.pframe) { free(e->spec[k]); e->spec[k] = NULL; }
-  if (pattern && !e->pattern_set) {
+  if (pattern && !e->pattern_set && !e->ep.pshow) {
     /* src/fft3d_engine.h:485-504 */
     int pf = e->ep.pframe;
     const cplx* ps = spectrum_of(c, pl, pf);
@@ -693,6 +694,31 @@ static void engine_frame(oclip* c, int pl, int n, uint8_t* dst, int dstride) {
     e->ep.px = px; e->ep.py = py;
     set_pattern(e, ps, px, py);
     e->pattern_set = 1;
+  }
+  if (pattern && e->ep.pshow) {
+    /* Pattern display, src/fft3d_engine.h:505-565: the pattern block of
+     * this frame (searched unless px/py are given) reconstructed alone with
+     * analysis and synthesis windows set to 1 -- permanently, so calls after
+     * the first also search with flat windows -- and the plane centred for
+     * every non-RGB plane, luma included. */
+    cplx* X = xcalloc(ssz, sizeof(cplx));
+    double* blk = xcalloc((size_t)e->nb * bh * bw, 8);
+    plane_spectra(e, src, sstride, X, e->chroma);
+    int px = e->ep.px, py = e->ep.py;
+    if (px == 0 && py == 0) find_pattern_block(e, X, &px, &py);
+    set_pattern(e, X, px, py);
+    for (int i = 0; i < e->ep.ow; i++) e->wanxl[i] = e->wanxr[i] = e->wsynxl[i] = e->wsynxr[i] = 1.0f;
+    for (int i = 0; i < e->ep.oh; i++) e->wanyl[i] = e->wanyr[i] = e->wsynyl[i] = e->wsynyr[i] = 1.0f;
+    e->pshow_flat = 1;
+    int centre = e->fmt.family != 2;
+    plane_spectra(e, src, sstride, X, centre);
+    int keep = py * e->nox + px;
+    for (int b = 0; b < e->nb; b++)
+      if (b != keep) memset(X + (size_t)b * bh * ow, 0, (size_t)bh * ow * sizeof(cplx));
+    for (int b = 0; b < e->nb; b++) c2r2d(X + (size_t)b * bh * ow, blk + (size_t)b * bh * bw, bw, bh, e->tmp);
+    synth_overlap(e, blk, dst, dstride, centre);
+    free(X); free(blk);
+    return;
   }
   cplx* out = xcalloc(ssz, sizeof(cplx));
   double* blocks = xcalloc((size_t)e->nb * bh * bw, 8);
@@ -820,7 +846,6 @@ void* nf3d_oracle_create(const char* args, int family, int sample_type, int bits
   c->crop = p.l > 0 || p.r > 0 || p.t > 0 || p.b > 0;
   if (p.l + p.r >= width) { set_err(err, errlen, "Width cannot be cropped to zero or below"); nf3d_oracle_free(c); return NULL; }
   if (p.t + p.b >= height) { set_err(err, errlen, "Height cannot be cropped to zero or below"); nf3d_oracle_free(c); return NULL; }
-  if (p.pshow) { set_err(err, errlen, "pshow is not restated by the oracle"); nf3d_oracle_free(c); return NULL; }
   for (int i = 0; i < 4; i++) c->process[i] = 2;
   if (!have_planes) { p.nplanes_list = 3; p.planes[0] = 0; p.planes[1] = 1; p.planes[2] = 2; }
   for (int i = 0; i < p.nplanes_list; i++)

@@ -1,7 +1,8 @@
 """Writes the golden vectors of tests/golden/*.npz from the compiled reference.
 
 Run in a container that has the reference tree (oracle/build_ref.sh builds
-oracle/_ref/libnf3d_ref.so from it): ``python tests/golden/make_golden.py``.
+oracle/_ref/libnf3d_ref.so from it): ``python tests/golden/make_golden.py``
+(all fixtures) or ``... make_golden.py <name> ...`` (the named ones).
 Each fixture holds the seeded input frames, the filter arguments and the
 reference's output frames (opt=1, the C kernels; ncpu=1; MKL sequential
 FFTW3 interface as the FFT backend).  The GPU box never runs the reference:
@@ -46,6 +47,8 @@ CASES = [
     ("yuv420p10_planes", "yuv", 10, False, 96, 64, 5, "sigma=3;bt=3;bw=16;bh=16;planes=0,2"),
     ("yuv420p16_kalman", "yuv", 16, False, 96, 64, 5, "sigma=3;bt=0;bw=32;bh=16;ow=8;oh=4;kratio=1.5"),
     ("rgbps_float", "rgb", 32, True, 64, 48, 4, "sigma=3;bt=3;bw=16;bh=16"),
+    ("g8_pshow", "gray", 8, False, 160, 120, 3, "sigma=3;bt=3;bw=16;bh=16;pfactor=1;pshow=1"),
+    ("yuv420p8_pshow_pxpy", "yuv", 8, False, 96, 64, 3, "sigma=3;bt=1;bw=16;bh=16;pfactor=1;pshow=1;px=3;py=2"),
 ]
 
 # The CPU-runnable workload, 5 frames: Gray8 640x360, bt=3, sigma=2, 48/16.
@@ -69,10 +72,15 @@ def save(name, clip, args, outs):
 
 def main():
     os.environ.setdefault("MKL_THREADING_LAYER", "SEQUENTIAL")
+    only = set(sys.argv[1:])  # optional: names of the fixtures to (re)write
     for name, fam, bits, isf, w, h, f, args in CASES:
+        if only and name not in only:
+            continue
         clip = make_clip(fam, bits, isf, width=w, height=h, nframes=f, seed=7)
         save(name, clip, args, run_ref(clip, args))
         print("wrote", name)
+    if only:
+        return
     from neo_fft3d_amd.synth import SEED_BASE, gratings_clip
     g = gratings_clip(640, 360, 5, SEED_BASE + 0, noise=4.0)
     from harness import Clip

@@ -69,6 +69,8 @@ MODES = [
     "sigma=3;bt=3;bw=16;bh=32;ow=4;oh=8",
     "sigma=2;bt=3;bw=48;bh=48;ow=16;oh=16",
     "sigma=2;bt=1;bw=64;bh=64;ow=16;oh=16",
+    "sigma=3;bt=3;bw=16;bh=16;pfactor=1;pshow=1",
+    "sigma=3;bt=1;bw=16;bh=16;pfactor=1;pshow=1;px=4;py=3",
 ]
 
 
