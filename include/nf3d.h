@@ -94,6 +94,34 @@ int nf3d_engine_get_frame_device(nf3d_engine* e, int n, const nf3d_frame_ref* fr
 int nf3d_engine_run_device(nf3d_engine* e, int first, int count, const void* const* src, int src_stride,
                            void* const* dst, int dst_stride, void* stream, char* err, size_t errlen);
 
+/* Batched form of nf3d_engine_run_device (no reference counterpart): one
+ * item per engine -- the planes of a clip, or of several clips -- each with
+ * its own frame range and planes.  The result is that of calling
+ * nf3d_engine_run_device for every item, bit for bit; engines with the same
+ * block shape, mode (bt) and sharpen/dehalo case share their frame-loop
+ * launches (up to 6 engines per launch), all on `stream` (NULL: the first
+ * item's engine's stream).  An engine may appear once per call. */
+typedef struct {
+  nf3d_engine* engine;
+  int first, count;
+  const void* const* src;
+  int src_stride;
+  void* const* dst;
+  int dst_stride;
+} nf3d_run_item;
+int nf3d_run_device_batch(const nf3d_run_item* items, int n, void* stream, char* err, size_t errlen);
+
+/* The work of one nf3d_run_device_batch call captured as a HIP graph on
+ * `stream` (a non-null hipStream_t) instead of being run; nf3d_graph_launch
+ * replays it on a stream (each replay filters the same frames again, as a
+ * repeated batch call would).  Capture needs the engines' pattern estimate
+ * already made (a first, uncaptured call does it) since it synchronises. */
+typedef struct nf3d_graph nf3d_graph;
+int nf3d_run_device_batch_graph(const nf3d_run_item* items, int n, void* stream, nf3d_graph** graph, char* err,
+                                size_t errlen);
+int nf3d_graph_launch(nf3d_graph* graph, void* stream);
+void nf3d_graph_destroy(nf3d_graph* graph);
+
 /* Replaces FFT3DEngine::CacheRefresh (src/fft3d_engine.h:736-740): 1 if the
  * spectrum of frame n is held in the HBM spectra ring (and marks it recent). */
 int nf3d_engine_cache_refresh(nf3d_engine* e, int n);

@@ -54,6 +54,13 @@ class FrameRef(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int), ("data", ctypes.c_void_p), ("stride", ctypes.c_int)]
 
 
+class RunItem(ctypes.Structure):
+    """nf3d_run_item: one engine's frame range in a batched call."""
+    _fields_ = [("engine", ctypes.c_void_p), ("first", ctypes.c_int), ("count", ctypes.c_int),
+                ("src", ctypes.POINTER(ctypes.c_void_p)), ("src_stride", ctypes.c_int),
+                ("dst", ctypes.POINTER(ctypes.c_void_p)), ("dst_stride", ctypes.c_int)]
+
+
 _lib = None
 
 
@@ -73,6 +80,12 @@ def lib() -> ctypes.CDLL:
                                                    ctypes.c_char_p, sz]
         L.nf3d_engine_run_device.argtypes = [vp, ip, ip, ctypes.POINTER(vp), ip, ctypes.POINTER(vp), ip, vp,
                                              ctypes.c_char_p, sz]
+        L.nf3d_run_device_batch.argtypes = [ctypes.POINTER(RunItem), ip, vp, ctypes.c_char_p, sz]
+        L.nf3d_run_device_batch_graph.argtypes = [ctypes.POINTER(RunItem), ip, vp, ctypes.POINTER(vp),
+                                                  ctypes.c_char_p, sz]
+        L.nf3d_graph_launch.argtypes = [vp, vp]
+        L.nf3d_graph_destroy.argtypes = [vp]
+        L.nf3d_graph_destroy.restype = None
         L.nf3d_engine_cache_refresh.argtypes = [vp, ip]
         L.nf3d_engine_geometry.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
         L.nf3d_engine_profile.argtypes = [vp, ip]
@@ -154,6 +167,55 @@ class Engine:
         if getattr(self, "h", None):
             lib().nf3d_engine_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _run_items(items):
+    """items: (engine, first, count, src_ptrs, src_stride, dst_ptrs, dst_stride) tuples."""
+    arr = (RunItem * len(items))()
+    keep = []
+    for k, (eng, first, count, sp, ss, dp, ds) in enumerate(items):
+        s = (ctypes.c_void_p * len(sp))(*sp)
+        d = (ctypes.c_void_p * len(dp))(*dp)
+        keep += [s, d]
+        arr[k] = RunItem(eng.h, first, count, ctypes.cast(s, ctypes.POINTER(ctypes.c_void_p)), ss,
+                         ctypes.cast(d, ctypes.POINTER(ctypes.c_void_p)), ds)
+    return arr, keep
+
+
+def run_device_batch(items, stream: int = 0):
+    """nf3d_run_device_batch: several engines' device ranges, launches shared."""
+    arr, keep = _run_items(items)
+    err = ctypes.create_string_buffer(512)
+    if lib().nf3d_run_device_batch(arr, len(items), ctypes.c_void_p(stream), err, 512) != 0:
+        raise Nf3dError(err.value.decode())
+
+
+class Graph:
+    """A captured nf3d_run_device_batch call (nf3d_run_device_batch_graph)."""
+
+    def __init__(self, items, stream: int):
+        arr, keep = _run_items(items)
+        g = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(512)
+        if lib().nf3d_run_device_batch_graph(arr, len(items), ctypes.c_void_p(stream), ctypes.byref(g),
+                                             err, 512) != 0:
+            raise Nf3dError(err.value.decode())
+        self.g = g
+
+    def launch(self, stream: int):
+        if lib().nf3d_graph_launch(self.g, ctypes.c_void_p(stream)) != 0:
+            raise Nf3dError("graph launch failed")
+
+    def close(self):
+        if getattr(self, "g", None):
+            lib().nf3d_graph_destroy(self.g)
+            self.g = None
 
     def __del__(self):
         try:

@@ -751,97 +751,148 @@ struct nf3d_engine {
     return ep.dehalo == 0.0f ? nf3d::SHP_SHARPEN : nf3d::SHP_GENERAL;
   }
 
-  // Frames [first, first + count) of a device-resident clip, in order.  The
-  // frames whose filter window is whole (every frame for bt = 1 / -1, all but
-  // the first for Kalman, btcur == bt for Wiener 3D) go through k_run, up to
-  // kRunFrames per launch; the others (clip ends) through the per-frame path.
-  void run_range(int first, int count, const void* const* src, long long src_pitch, void* const* dst,
-                 long long dst_pitch, hipStream_t s) {
-    check(hipSetDevice(device), "hipSetDevice");
+  // Plan of a device range: frames [first, first + count) of a device-resident
+  // clip, in order.  The frames whose filter window is whole (every frame for
+  // bt = 1 / -1, all but the first for Kalman, btcur == bt for Wiener 3D),
+  // frames [a, b], go through k_run in launches of up to kRunFrames frames
+  // (`launches`, one job record each); the others (clip ends) through the
+  // per-frame path, before (prologue) and after (epilogue) them.
+  struct RangePlan {
+    int first = 0, count = 0, a = 0, b = -1;
+    int mode = 0, shp = 0, from = 0, to = 0;
+    bool run = false;  // false: every frame through the per-frame path
+    const void* const* src = nullptr;
+    void* const* dst = nullptr;
+    long long src_pitch = 0, dst_pitch = 0;
+    std::vector<RunLaunch> launches;
+  };
+
+  RangePlan plan_range(int first, int count, const void* const* src, long long src_pitch, void* const* dst,
+                       long long dst_pitch) {
     for (int n = first; n < first + count; n++)
       if (n < 0 || n >= frames) fail("frame " + std::to_string(n) + " outside the clip");
+    RangePlan P;
+    P.first = first;
+    P.count = count;
+    P.src = src;
+    P.dst = dst;
+    P.src_pitch = src_pitch;
+    P.dst_pitch = dst_pitch;
     int lo = 0, hi = frames - 1;
-    int mode = nf3d::MODE_W2D;
+    P.mode = nf3d::MODE_W2D;
     if (ep.bt >= 2) {
-      mode = nf3d::MODE_W3D;
+      P.mode = nf3d::MODE_W3D;
       lo = ep.bt / 2;
       hi = frames - 1 - (ep.bt - 1) / 2;
     } else if (ep.bt == 0) {
-      mode = nf3d::MODE_KALMAN;
+      P.mode = nf3d::MODE_KALMAN;
       lo = 1;
     } else if (ep.bt == -1) {
-      mode = nf3d::MODE_SHARPEN;
+      P.mode = nf3d::MODE_SHARPEN;
     }
-    const int shp = shp_kind();
-    const int a = std::max(first, lo), b = std::min(first + count - 1, hi);
+    P.shp = shp_kind();
+    P.a = std::max(first, lo);
+    P.b = std::min(first + count - 1, hi);
+    P.from = P.mode == nf3d::MODE_W3D ? -(ep.bt / 2) : 0;
+    P.to = P.mode == nf3d::MODE_W3D ? (ep.bt - 1) / 2 : 0;
     // k_run keeps 32-bit offsets (source rows, destination and seam
     // elements); the sharpen-only mode without sharpen or dehalo is a plain
-    // round trip left to the per-frame path
+    // round trip, and the pattern display a different path: both are left
+    // to the per-frame path
     const bool fits = (long long)(tc + Hc) * src_pitch < (1LL << 31) &&
                       (long long)(tc + Hc) * dst_pitch < (1LL << 31) && 4LL * Hc * Wc < (1LL << 31) &&
                       dst_pitch % pix == 0;
     const bool pshow = ep.pfactor != 0 && ep.pshow;
-    if (a > b || !fits || pshow || (mode == nf3d::MODE_SHARPEN && shp == nf3d::SHP_NONE)) {
-      for (int n = first; n < first + count; n++) run_one(n, src, src_pitch, dst, dst_pitch, s);
-      return;
-    }
-    for (int n = first; n < a; n++) run_one(n, src, src_pitch, dst, dst_pitch, s);
-    if (ep.pfactor != 0 && !pattern_set) {
-      const int pf = std::max(0, std::min(ep.pframe, frames - 1));
-      nf3d_frame_ref r{pf, src[pf], (int)src_pitch};
-      estimate_pattern(Call{&r, 1, true, s});
-    }
-    const int from = mode == nf3d::MODE_W3D ? -(ep.bt / 2) : 0;
-    const int to = mode == nf3d::MODE_W3D ? (ep.bt - 1) / 2 : 0;
-    if (mode == nf3d::MODE_W3D) {
-      if (ep.pfactor != 0) ensure_pattern3d(ep.bt, s);
-      // the window's older frames of the first step, into slot (frame % bt)
-      for (int k = a + from; k < a + to; k++) {
-        PlaneJob F = base_job();
-        F.src = static_cast<const uint8_t*>(src[k]);
-        F.src_pitch = src_pitch;
-        F.spec_out = slot_ptr(k % ep.bt);
-        launch(nf3d::MODE_FWD, 0, F, s);
-      }
-    }
-    // launches of equal length (within one frame)
+    P.run = P.a <= P.b && fits && !pshow && !(P.mode == nf3d::MODE_SHARPEN && P.shp == nf3d::SHP_NONE);
+    if (!P.run) return P;
+    // launches of equal length (within one frame), two halves of seam
+    // buffers in turn
     const size_t seam_frame = (size_t)4 * Hc * Wc;
     if (!d_seam_run) d_seam_run = dev_alloc<float>(2 * kRunFrames * seam_frame);
-    const int total = b - a + 1;
+    const int total = P.b - P.a + 1;
     const int nl = (total + kRunFrames - 1) / kRunFrames;
-    for (int j = 0, m0 = a; j < nl; j++) {
+    for (int j = 0, m0 = P.a; j < nl; j++) {
       const int nf = total / nl + (j >= nl - total % nl ? 1 : 0);
       RunLaunch R{};
       R.J = base_job();
-      R.J.sigma2 = (float)(mode == nf3d::MODE_W3D ? ep.bt : 1) * ep.sigma * ep.sigma / norm;
+      R.J.sigma2 = (float)(P.mode == nf3d::MODE_W3D ? ep.bt : 1) * ep.sigma * ep.sigma / norm;
       R.J.seam = d_seam_run + (size_t)(j & 1) * kRunFrames * seam_frame;
       R.J.src_pitch = src_pitch;
       R.J.dst_pitch = dst_pitch;
       for (int i = 0; i < nf; i++) {
-        R.src[i] = static_cast<const uint8_t*>(src[m0 + i + to]);
+        R.src[i] = static_cast<const uint8_t*>(src[m0 + i + P.to]);
         R.dst[i] = static_cast<uint8_t*>(dst[m0 + i]);
       }
       R.ring = d_ring;
       R.spec_elems = (long long)spec_elems;
       R.m0 = m0;
       R.nframes = nf;
-      R.fpw = (mode == nf3d::MODE_W3D || mode == nf3d::MODE_KALMAN) ? nf : 1;
+      R.fpw = (P.mode == nf3d::MODE_W3D || P.mode == nf3d::MODE_KALMAN) ? nf : 1;
       R.ntiles = noy * ntx;
-      auto ev = timer(true, nf, s);
-      check(nf3d::launch_run(mode, ep.bt, shp, ep.bw, ep.bh, R, s), "frame-loop kernel");
-      check(nf3d::launch_seam_run(ep.bw, ep.bh, R, s), "seam kernel");
-      if (ev.second) check(hipEventRecord(ev.second, s), "hipEventRecord");
+      P.launches.push_back(R);
       m0 += nf;
     }
-    if (mode == nf3d::MODE_W3D) {
+    return P;
+  }
+
+  // Frames before the frame-loop launches (or all, without them), the
+  // pattern estimate, the Wiener-3D ring slots of the first window.
+  void run_prologue(const RangePlan& P, hipStream_t s) {
+    check(hipSetDevice(device), "hipSetDevice");
+    const int stop = P.run ? P.a : P.first + P.count;
+    for (int n = P.first; n < stop; n++) run_one(n, P.src, P.src_pitch, P.dst, P.dst_pitch, s);
+    if (!P.run) return;
+    if (ep.pfactor != 0 && !pattern_set) {
+      const int pf = std::max(0, std::min(ep.pframe, frames - 1));
+      nf3d_frame_ref r{pf, P.src[pf], (int)P.src_pitch};
+      estimate_pattern(Call{&r, 1, true, s});
+    }
+    if (P.mode == nf3d::MODE_W3D) {
+      if (ep.pfactor != 0) ensure_pattern3d(ep.bt, s);
+      // the window's older frames of the first step, into slot (frame % bt)
+      for (int k = P.a + P.from; k < P.a + P.to; k++) {
+        PlaneJob F = base_job();
+        F.src = static_cast<const uint8_t*>(P.src[k]);
+        F.src_pitch = P.src_pitch;
+        F.spec_out = slot_ptr(k % ep.bt);
+        launch(nf3d::MODE_FWD, 0, F, s);
+      }
+    }
+  }
+
+  // Ring bookkeeping after the frame-loop launches, then the frames after them.
+  void run_epilogue(const RangePlan& P, hipStream_t s) {
+    if (!P.run) return;
+    if (P.mode == nf3d::MODE_W3D) {
       // slot (k % bt) now holds frame k for the last bt frames transformed
       std::fill(ring_frame.begin(), ring_frame.end(), -1);
-      for (int k = std::max(0, b + to - ep.bt + 1); k <= b + to; k++) {
+      for (int k = std::max(0, P.b + P.to - ep.bt + 1); k <= P.b + P.to; k++) {
         ring_frame[k % ep.bt] = k;
         ring_tick[k % ep.bt] = ++tick;
       }
     }
-    for (int n = b + 1; n < first + count; n++) run_one(n, src, src_pitch, dst, dst_pitch, s);
+    for (int n = P.b + 1; n < P.first + P.count; n++) run_one(n, P.src, P.src_pitch, P.dst, P.dst_pitch, s);
+  }
+
+  void run_range(int first, int count, const void* const* src, long long src_pitch, void* const* dst,
+                 long long dst_pitch, hipStream_t s) {
+    check(hipSetDevice(device), "hipSetDevice");
+    RangePlan P = plan_range(first, count, src, src_pitch, dst, dst_pitch);
+    run_prologue(P, s);
+    for (const RunLaunch& R : P.launches) {
+      nf3d::RunBatch B{};
+      B.job[0] = R;
+      B.job[0].wg_begin = 0;
+      B.njobs = 1;
+      B.total_wg = R.ntiles * ((R.nframes + R.fpw - 1) / R.fpw);
+      B.max_hc = Hc;
+      B.max_frames = R.nframes;
+      auto ev = timer(true, R.nframes, s);
+      check(nf3d::launch_run(P.mode, ep.bt, P.shp, ep.bw, ep.bh, B, s), "frame-loop kernel");
+      check(nf3d::launch_seam_run(ep.bw, ep.bh, B, s), "seam kernel");
+      if (ev.second) check(hipEventRecord(ev.second, s), "hipEventRecord");
+    }
+    run_epilogue(P, s);
   }
 
   void get_frame_host(int n, const nf3d_frame_ref* frames_, int nframes, void* dst, int dst_stride) {
@@ -951,6 +1002,167 @@ int nf3d_engine_run_device(nf3d_engine* e, int first, int count, const void* con
     return -1;
   }
   return 0;
+}
+
+namespace {
+
+// The batch: plans of every item, prologues, the frame-loop launches grouped
+// by (block shape, mode, bt, sharpen case) and launch index, epilogues.
+void run_batch(const nf3d_run_item* items, int n, hipStream_t s, bool timed) {
+  std::vector<nf3d_engine::RangePlan> plans(n);
+  for (int i = 0; i < n; i++) {
+    const nf3d_run_item& it = items[i];
+    plans[i] = it.engine->plan_range(it.first, it.count, it.src, it.src_stride, it.dst, it.dst_stride);
+  }
+  for (int i = 0; i < n; i++) items[i].engine->run_prologue(plans[i], s);
+  std::vector<bool> done(n, false);
+  for (int i = 0; i < n; i++) {
+    if (done[i] || !plans[i].run) continue;
+    nf3d_engine* e0 = items[i].engine;
+    std::vector<int> group;
+    for (int k = i; k < n; k++) {
+      nf3d_engine* e = items[k].engine;
+      if (!done[k] && plans[k].run && e->ep.bw == e0->ep.bw && e->ep.bh == e0->ep.bh && e->ep.bt == e0->ep.bt &&
+          plans[k].mode == plans[i].mode && plans[k].shp == plans[i].shp) {
+        group.push_back(k);
+        done[k] = true;
+      }
+    }
+    size_t nl = 0;
+    for (int k : group) nl = std::max(nl, plans[k].launches.size());
+    for (size_t l = 0; l < nl; l++) {
+      std::vector<int> members;
+      for (int k : group)
+        if (l < plans[k].launches.size()) members.push_back(k);
+      for (size_t c0 = 0; c0 < members.size(); c0 += nf3d::RUN_MAX_JOBS) {
+        nf3d::RunBatch B{};
+        std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
+        for (size_t c = c0; c < members.size() && c < c0 + nf3d::RUN_MAX_JOBS; c++) {
+          const int k = members[c];
+          nf3d_engine* e = items[k].engine;
+          RunLaunch& R = B.job[B.njobs++];
+          R = plans[k].launches[l];
+          R.wg_begin = B.total_wg;
+          B.total_wg += R.ntiles * ((R.nframes + R.fpw - 1) / R.fpw);
+          B.max_hc = std::max(B.max_hc, e->Hc);
+          B.max_frames = std::max(B.max_frames, R.nframes);
+          evs.push_back(e->timer(timed, R.nframes, s));
+        }
+        check(nf3d::launch_run(plans[i].mode, e0->ep.bt, plans[i].shp, e0->ep.bw, e0->ep.bh, B, s),
+              "frame-loop kernel");
+        check(nf3d::launch_seam_run(e0->ep.bw, e0->ep.bh, B, s), "seam kernel");
+        for (auto& ev : evs)
+          if (ev.second) check(hipEventRecord(ev.second, s), "hipEventRecord");
+      }
+    }
+  }
+  for (int i = 0; i < n; i++) items[i].engine->run_epilogue(plans[i], s);
+}
+
+// Locks every engine of a batch (address order), after checking the items.
+std::vector<std::unique_lock<std::mutex>> lock_batch(const nf3d_run_item* items, int n) {
+  if (n < 1) fail("empty batch");
+  std::vector<nf3d_engine*> es;
+  for (int i = 0; i < n; i++) {
+    if (!items[i].engine) fail("batch item without an engine");
+    es.push_back(items[i].engine);
+  }
+  std::sort(es.begin(), es.end());
+  if (std::adjacent_find(es.begin(), es.end()) != es.end()) fail("an engine appears twice in the batch");
+  for (nf3d_engine* e : es)
+    if (e->device != es[0]->device) fail("batch engines on different devices");
+  std::vector<std::unique_lock<std::mutex>> locks;
+  for (nf3d_engine* e : es) locks.emplace_back(e->mu);
+  check(hipSetDevice(es[0]->device), "hipSetDevice");
+  return locks;
+}
+
+}  // namespace
+
+struct nf3d_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+int nf3d_run_device_batch(const nf3d_run_item* items, int n, void* stream, char* err, size_t errlen) {
+  try {
+    auto locks = lock_batch(items, n);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : items[0].engine->stream;
+    run_batch(items, n, s, true);
+  } catch (const Failure& f) {
+    copy_err(err, errlen, f.msg);
+    return -1;
+  }
+  return 0;
+}
+
+int nf3d_run_device_batch_graph(const nf3d_run_item* items, int n, void* stream, nf3d_graph** graph, char* err,
+                                size_t errlen) {
+  *graph = nullptr;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  bool capturing = false;
+  try {
+    if (!s) fail("graph capture needs a stream");
+    auto locks = lock_batch(items, n);
+    for (int i = 0; i < n; i++) {
+      nf3d_engine* e = items[i].engine;
+      if (e->ep.pfactor != 0 && (!e->pattern_set || (e->ep.bt >= 2 && !e->pattern3d_init)))
+        fail("the noise pattern is estimated on the first call, which cannot be captured: run the batch once first");
+      // buffers the launches use, allocated outside the capture
+      e->plan_range(items[i].first, items[i].count, items[i].src, items[i].src_stride, items[i].dst,
+                    items[i].dst_stride);
+    }
+    // no profiling events inside the graph
+    std::vector<bool> prof(n);
+    for (int i = 0; i < n; i++) {
+      prof[i] = items[i].engine->profiling;
+      items[i].engine->profiling = false;
+    }
+    struct Restore {
+      const nf3d_run_item* items;
+      std::vector<bool>& prof;
+      ~Restore() {
+        for (size_t i = 0; i < prof.size(); i++) items[i].engine->profiling = prof[i];
+      }
+    } restore{items, prof};
+    check(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+    capturing = true;
+    run_batch(items, n, s, false);
+    auto* g = new nf3d_graph;
+    capturing = false;
+    hipError_t e1 = hipStreamEndCapture(s, &g->graph);
+    if (e1 != hipSuccess) {
+      delete g;
+      check(e1, "hipStreamEndCapture");
+    }
+    hipError_t e2 = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e2 != hipSuccess) {
+      (void)hipGraphDestroy(g->graph);
+      delete g;
+      check(e2, "hipGraphInstantiate");
+    }
+    *graph = g;
+  } catch (const Failure& f) {
+    if (capturing) {
+      hipGraph_t dead = nullptr;
+      if (hipStreamEndCapture(s, &dead) == hipSuccess && dead) (void)hipGraphDestroy(dead);
+    }
+    copy_err(err, errlen, f.msg);
+    return -1;
+  }
+  return 0;
+}
+
+int nf3d_graph_launch(nf3d_graph* g, void* stream) {
+  if (!g || !g->exec) return -1;
+  return hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)) == hipSuccess ? 0 : -1;
+}
+
+void nf3d_graph_destroy(nf3d_graph* g) {
+  if (!g) return;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
 }
 
 int nf3d_engine_cache_refresh(nf3d_engine* e, int n) {

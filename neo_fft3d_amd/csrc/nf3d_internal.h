@@ -85,6 +85,7 @@ enum Shp : int {
   SHP_GENERAL = 3,  // dehalo != 0 (sharpen factor times dehalo factor; 1 exactly when sharpen == 0)
 };
 
+// One job of a frame-loop launch: a run of frames of one plane of one clip.
 struct RunLaunch {
   PlaneJob J;                           // geometry, tables, parameters; J.seam = per-frame seam buffers
   const uint8_t* src[RUN_MAX_FRAMES];   // plane transformed on chip at step i (frame m0+i, +(bt-1)/2 for Wiener 3D)
@@ -94,6 +95,18 @@ struct RunLaunch {
   int m0, nframes;
   int fpw;                              // consecutive frames filtered by one workgroup
   int ntiles;                           // tiles of the plane (noy * ntx)
+  int wg_begin;                         // first workgroup of this job in the launch
+};
+
+// Jobs of one launch (same block shape, mode, bt and sharpen case): the
+// planes of a clip, or of several clips.
+constexpr int RUN_MAX_JOBS = 6;
+struct RunBatch {
+  RunLaunch job[RUN_MAX_JOBS];
+  int njobs;
+  int total_wg;    // sum of the jobs' workgroups
+  int max_hc;      // largest plane height (seam grid)
+  int max_frames;  // largest nframes (seam grid)
 };
 
 // Pattern estimation (src/helper.cpp:46-132).
@@ -114,10 +127,10 @@ struct PatternJob {
 // launch_tile runs k_tile and, for the filtering modes, k_seam.
 hipError_t launch_tile(int mode, int bt, int onchip, int bw, int bh, const TileLaunch& l, hipStream_t s);
 hipError_t launch_pattern(const PatternJob& job, hipStream_t s);
-// k_run over L.nframes frames; `shp` is a Shp value.  launch_seam_run
-// finishes the seams of L's frames (after its k_run, same stream).
-hipError_t launch_run(int mode, int bt, int shp, int bw, int bh, const RunLaunch& l, hipStream_t s);
-hipError_t launch_seam_run(int bw, int bh, const RunLaunch& l, hipStream_t s);
+// k_run over the batch's jobs; `shp` is a Shp value.  launch_seam_run
+// finishes the seams of their frames (after the k_run, same stream).
+hipError_t launch_run(int mode, int bt, int shp, int bw, int bh, const RunBatch& b, hipStream_t s);
+hipError_t launch_seam_run(int bw, int bh, const RunBatch& b, hipStream_t s);
 bool block_shape_supported(int bw, int bh);
 int tile_blocks_per_wg(int bw, int bh);
 

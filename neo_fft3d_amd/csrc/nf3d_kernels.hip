@@ -65,8 +65,8 @@ __global__ void __launch_bounds__(256) k_pattern_set(PatternJob J) {
 
 #define X(W, H)                                                                                          \
   hipError_t launch_tile_##W##x##H(int mode, int bt, int onchip, const TileLaunch& l, hipStream_t s); \
-  hipError_t launch_run_##W##x##H(int mode, int bt, int shp, const RunLaunch& l, hipStream_t s); \
-  hipError_t launch_seam_run_##W##x##H(const RunLaunch& l, hipStream_t s);
+  hipError_t launch_run_##W##x##H(int mode, int bt, int shp, const RunBatch& l, hipStream_t s); \
+  hipError_t launch_seam_run_##W##x##H(const RunBatch& l, hipStream_t s);
 NF3D_SHAPES(X)
 #undef X
 
@@ -90,14 +90,14 @@ hipError_t launch_tile(int mode, int bt, int onchip, int bw, int bh, const TileL
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_run(int mode, int bt, int shp, int bw, int bh, const RunLaunch& l, hipStream_t s) {
+hipError_t launch_run(int mode, int bt, int shp, int bw, int bh, const RunBatch& l, hipStream_t s) {
 #define X(W, H) if (bw == W && bh == H) return launch_run_##W##x##H(mode, bt, shp, l, s);
   NF3D_SHAPES(X)
 #undef X
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_seam_run(int bw, int bh, const RunLaunch& l, hipStream_t s) {
+hipError_t launch_seam_run(int bw, int bh, const RunBatch& l, hipStream_t s) {
 #define X(W, H) if (bw == W && bh == H) return launch_seam_run_##W##x##H(l, s);
   NF3D_SHAPES(X)
 #undef X

@@ -25,10 +25,10 @@
 
 namespace nf3d {
 
-__device__ __forceinline__ const RunLaunch& kernarg_run(const RunLaunch& arg) {
+__device__ __forceinline__ const RunBatch& kernarg_batch(const RunBatch& arg) {
 #if defined(__HIP_DEVICE_COMPILE__)
   (void)arg;
-  return *(const RunLaunch*)__builtin_amdgcn_kernarg_segment_ptr();
+  return *(const RunBatch*)__builtin_amdgcn_kernarg_segment_ptr();
 #else
   return arg;
 #endif
@@ -37,15 +37,23 @@ __device__ __forceinline__ const RunLaunch& kernarg_run(const RunLaunch& arg) {
 // The same record through an offset the compiler cannot see through (zero at
 // run time): loads of launch fields inside a loop stay in the loop instead of
 // being hoisted into registers that live across it.
-__device__ __forceinline__ const RunLaunch& kernarg_run_opaque(const RunLaunch& arg) {
+__device__ __forceinline__ const RunBatch& kernarg_batch_opaque(const RunBatch& arg) {
 #if defined(__HIP_DEVICE_COMPILE__)
   (void)arg;
   int zero = 0;
   asm volatile("" : "+s"(zero));
-  return *(const RunLaunch*)((const char*)__builtin_amdgcn_kernarg_segment_ptr() + zero);
+  return *(const RunBatch*)((const char*)__builtin_amdgcn_kernarg_segment_ptr() + zero);
 #else
   return arg;
 #endif
+}
+
+// The job of a workgroup: the last one whose first workgroup it is past.
+__device__ __forceinline__ int batch_job(const RunBatch& B, int wg) {
+  int jb = 0;
+  for (int k = 1; k < B.njobs; k++)
+    if (wg >= B.job[k].wg_begin) jb = k;
+  return jb;
 }
 
 template <int I, typename X>
@@ -323,8 +331,10 @@ struct TabBuf {
 // mask keeps its old value in the other lanes, which would hold it live around
 // the frame loop.
 template <int BW, int BH, int NT, int MODE, int BT, int SHP>
-__global__ void __launch_bounds__(NT, 2) k_run(RunLaunch L_arg) {
-  const RunLaunch& L0 = kernarg_run(L_arg);
+__global__ void __launch_bounds__(NT, 2) k_run(RunBatch B_arg) {
+  const RunBatch& B0 = kernarg_batch(B_arg);
+  const int jb = batch_job(B0, blockIdx.x);
+  const RunLaunch& L0 = B0.job[jb];
   using S = TShape<BW, BH, NT>;
   constexpr int COLS = S::COLS, PAIRS = S::PAIRS, BPW = S::BPW, SR = S::SR, LSPEC = S::LSPEC;
   constexpr int SYNW = S::SYNW, LSYN = S::LSYN, SA = S::SA, LA = S::LA;
@@ -349,8 +359,9 @@ __global__ void __launch_bounds__(NT, 2) k_run(RunLaunch L_arg) {
   float* syn = reinterpret_cast<float*>(smem);
 
   // ---- tile and frames of this workgroup
-  const int chunk = blockIdx.x / L0.ntiles;
-  const int tile = blockIdx.x - chunk * L0.ntiles;
+  const int g = blockIdx.x - L0.wg_begin;
+  const int chunk = g / L0.ntiles;
+  const int tile = g - chunk * L0.ntiles;
   const int i0 = chunk * L0.fpw;
   const int i1 = min(L0.nframes, i0 + L0.fpw);
   const int by = tile / L0.J.ntx;
@@ -400,7 +411,7 @@ __global__ void __launch_bounds__(NT, 2) k_run(RunLaunch L_arg) {
     asm volatile("" : "+v"(t));
     int zo = 0;
     asm volatile("" : "+s"(zo));
-    const RunLaunch& L = kernarg_run_opaque(L_arg);
+    const RunLaunch& L = kernarg_batch_opaque(B_arg).job[jb];
     const PlaneJob& J = L.J;
     const float* wx = winx + zo;
     const float* wy = winy + zo;
@@ -632,12 +643,13 @@ __global__ void __launch_bounds__(NT, 2) k_run(RunLaunch L_arg) {
 // flight), the corner partials where a vertical seam crosses.  Other rows
 // finish only the columns of the vertical seams between tiles.
 template <int BW, int BH, int NT>
-__global__ void __launch_bounds__(256) k_seam_run(RunLaunch L_arg) {
-  const RunLaunch& L = kernarg_run(L_arg);
+__global__ void __launch_bounds__(256) k_seam_run(RunBatch B_arg) {
+  const RunLaunch& L = kernarg_batch(B_arg).job[blockIdx.z];
   const PlaneJob& J = L.J;
   constexpr int BPW = TShape<BW, BH, NT>::BPW;
   const int yo = blockIdx.x;
   const int fi = blockIdx.y;
+  if (yo >= J.Hc || fi >= L.nframes) return;
   const float* sm = J.seam + (size_t)fi * 4 * (size_t)J.seam_plane;
   uint8_t* dst = L.dst[fi];
   const int stepx = BW - J.ow, stepy = BH - J.oh;
@@ -688,11 +700,17 @@ __global__ void __launch_bounds__(256) k_seam_run(RunLaunch L_arg) {
 }
 
 template <int BW, int BH>
-hipError_t launch_run_shape(int mode, int bt, int shp, const RunLaunch& L, hipStream_t s) {
+hipError_t launch_run_shape(int mode, int bt, int shp, const RunBatch& L, hipStream_t s) {
   constexpr int NT = 128;
-  if (L.nframes < 1 || L.nframes > RUN_MAX_FRAMES || L.fpw < 1) return hipErrorInvalidValue;
-  const int chunks = (L.nframes + L.fpw - 1) / L.fpw;
-  dim3 grid(L.ntiles * chunks), block(NT);
+  if (L.njobs < 1 || L.njobs > RUN_MAX_JOBS) return hipErrorInvalidValue;
+  int wg = 0;
+  for (int k = 0; k < L.njobs; k++) {
+    const RunLaunch& R = L.job[k];
+    if (R.nframes < 1 || R.nframes > RUN_MAX_FRAMES || R.fpw < 1 || R.wg_begin != wg) return hipErrorInvalidValue;
+    wg += R.ntiles * ((R.nframes + R.fpw - 1) / R.fpw);
+  }
+  if (wg != L.total_wg) return hipErrorInvalidValue;
+  dim3 grid(L.total_wg), block(NT);
 #define NF3D_RUN(M, B, H) hipLaunchKernelGGL((k_run<BW, BH, NT, M, B, H>), grid, block, 0, s, L)
 #define NF3D_RUN_SHP(M, B)                                   \
   switch (shp) {                                             \
@@ -726,9 +744,9 @@ hipError_t launch_run_shape(int mode, int bt, int shp, const RunLaunch& L, hipSt
 }
 
 template <int BW, int BH>
-hipError_t launch_seam_run_shape(const RunLaunch& L, hipStream_t s) {
-  if (L.nframes < 1 || L.nframes > RUN_MAX_FRAMES) return hipErrorInvalidValue;
-  dim3 sgrid(L.J.Hc, L.nframes);
+hipError_t launch_seam_run_shape(const RunBatch& L, hipStream_t s) {
+  if (L.njobs < 1 || L.njobs > RUN_MAX_JOBS || L.max_frames < 1 || L.max_frames > RUN_MAX_FRAMES) return hipErrorInvalidValue;
+  dim3 sgrid(L.max_hc, L.max_frames, L.njobs);
   hipLaunchKernelGGL((k_seam_run<BW, BH, 128>), sgrid, dim3(256), 0, s, L);
   return hipGetLastError();
 }
