@@ -165,6 +165,56 @@ def gpu_run(args):
     }
 
 
+def precision_form(args):
+    """DFT-as-GEMM precision form (nf3d_dft2d_mfma): 32x32 block DFTs of a
+    3840x2160 plane's block count (32/16 blocks), forward then inverse, on the
+    matrix cores with FP16 hi/lo split operands; error against float64."""
+    import numpy as np
+    import torch
+
+    from neo_fft3d_amd import lib
+    L = lib()
+    L.nf3d_dft2d_mfma.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_float, ctypes.c_void_p]
+    nox, noy = plane_geometry(3840, 2160, 32, 32, 16, 16)
+    nb = nox * noy
+    rng = np.random.default_rng(5)
+    x = rng.uniform(0, 1023, size=(nb, 32, 32)).astype(np.float32)
+    d_x = torch.from_numpy(x).cuda()
+    d_z = torch.empty((nb, 32, 17, 2), dtype=torch.float32, device="cuda")
+    d_y = torch.empty_like(d_x)
+
+    def once():
+        assert L.nf3d_dft2d_mfma(ctypes.c_void_p(d_x.data_ptr()), ctypes.c_void_p(d_z.data_ptr()), nb, 32, 0,
+                                 ctypes.c_float(1.0), None) == 0
+        assert L.nf3d_dft2d_mfma(ctypes.c_void_p(d_z.data_ptr()), ctypes.c_void_p(d_y.data_ptr()), nb, 32, 1,
+                                 ctypes.c_float(1.0 / 32), None) == 0
+    once()
+    torch.cuda.synchronize()
+    reps = 20
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        once()
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    sub = 256
+    want = np.fft.rfft2(x[:sub].astype(np.float64))
+    z = d_z[:sub].cpu().numpy()
+    got = z[..., 0] + 1j * z[..., 1]
+    fwd_err = float(np.abs(got - want).max() / np.abs(want).max())
+    back = d_y[:sub].cpu().numpy() / 1024.0
+    rt_err = float(np.abs(back - x[:sub]).max())
+    # FP16 MACs per block and direction: 3 split products x (32*64*32 + 2*32*64*32)
+    flops = 2 * 2 * 3 * (32 * 64 * 32 + 2 * 32 * 64 * 32) * nb
+    tflops = flops / (ms * 1e-3) / 1e12
+    return {"blocks": nb, "ms_forward_plus_inverse": round(ms, 4), "blocks_per_s": round(nb / (ms * 1e-3)),
+            "mfma_tflops_fp16": round(tflops, 1), "mfma_peak_tflops_fp16": 2500.0,
+            "frac": round(tflops / 2500.0, 4), "forward_max_err_rel_fp64": fwd_err,
+            "round_trip_max_err_10bit": rt_err}
+
+
 # ------------------------------------------------------------- CPU baseline
 def cpu_worker(args):
     """Runs in a child process (MKL threading layer set before it loads)."""
@@ -248,6 +298,8 @@ def main():
     ap.add_argument("--frames", type=int, default=300)
     ap.add_argument("--bt", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-precision-form", action="store_true",
+                    help="skip the DFT-as-GEMM (MFMA, FP16 split) transform measurement")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run one device copy of MIB MiB first (a reference kernel for traffic counters)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -276,6 +328,8 @@ def main():
         "roofline": r["roofline"],
         "psnr_db": r["psnr_db"],
     }
+    if not args.no_precision_form:
+        line["precision_form"] = precision_form(args)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args)
         cb = line["cpu_baseline"]["value"]

@@ -122,6 +122,18 @@ int nf3d_run_device_batch_graph(const nf3d_run_item* items, int n, void* stream,
 int nf3d_graph_launch(nf3d_graph* graph, void* stream);
 void nf3d_graph_destroy(nf3d_graph* graph);
 
+/* DFT-as-GEMM precision form (no reference counterpart): unnormalised 2D
+ * DFTs of n x n real blocks (n = 32) as dense DFT-matrix products on the
+ * matrix cores, FP16 hi/lo split operands (hi*hi + hi*lo + lo*hi) with FP32
+ * accumulation.  Forward: in [nblocks][n][n] float -> out [nblocks][n][n/2+1]
+ * complex (FFTW r2c layout); inverse: in [nblocks][n][n/2+1] complex -> out
+ * [nblocks][n][n] float (FFTW c2r: the imaginary parts of columns 0 and n/2
+ * are ignored).  `scale`, a power of two, multiplies the input before it is
+ * split so every operand stays in FP16 range (|scale * x| < 32768 forward,
+ * < 65504 inverse); the output is corrected exactly.  Device pointers; the
+ * work is enqueued on `stream`.  0 on success. */
+int nf3d_dft2d_mfma(const void* in, void* out, int nblocks, int n, int inverse, float scale, void* stream);
+
 /* Replaces FFT3DEngine::CacheRefresh (src/fft3d_engine.h:736-740): 1 if the
  * spectrum of frame n is held in the HBM spectra ring (and marks it recent). */
 int nf3d_engine_cache_refresh(nf3d_engine* e, int n);

@@ -874,27 +874,6 @@ struct nf3d_engine {
     for (int n = P.b + 1; n < P.first + P.count; n++) run_one(n, P.src, P.src_pitch, P.dst, P.dst_pitch, s);
   }
 
-  void run_range(int first, int count, const void* const* src, long long src_pitch, void* const* dst,
-                 long long dst_pitch, hipStream_t s) {
-    check(hipSetDevice(device), "hipSetDevice");
-    RangePlan P = plan_range(first, count, src, src_pitch, dst, dst_pitch);
-    run_prologue(P, s);
-    for (const RunLaunch& R : P.launches) {
-      nf3d::RunBatch B{};
-      B.job[0] = R;
-      B.job[0].wg_begin = 0;
-      B.njobs = 1;
-      B.total_wg = R.ntiles * ((R.nframes + R.fpw - 1) / R.fpw);
-      B.max_hc = Hc;
-      B.max_frames = R.nframes;
-      auto ev = timer(true, R.nframes, s);
-      check(nf3d::launch_run(P.mode, ep.bt, P.shp, ep.bw, ep.bh, B, s), "frame-loop kernel");
-      check(nf3d::launch_seam_run(ep.bw, ep.bh, B, s), "seam kernel");
-      if (ev.second) check(hipEventRecord(ev.second, s), "hipEventRecord");
-    }
-    run_epilogue(P, s);
-  }
-
   void get_frame_host(int n, const nf3d_frame_ref* frames_, int nframes, void* dst, int dst_stride) {
     Call c{frames_, nframes, false, stream};
     std::fill(stage_frame.begin(), stage_frame.end(), -1);
@@ -991,19 +970,6 @@ int nf3d_engine_get_frame_device(nf3d_engine* e, int n, const nf3d_frame_ref* fr
   return 0;
 }
 
-int nf3d_engine_run_device(nf3d_engine* e, int first, int count, const void* const* src, int src_stride,
-                           void* const* dst, int dst_stride, void* stream, char* err, size_t errlen) {
-  std::lock_guard<std::mutex> lock(e->mu);
-  try {
-    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : e->stream;
-    e->run_range(first, count, src, src_stride, dst, dst_stride, s);
-  } catch (const Failure& f) {
-    copy_err(err, errlen, f.msg);
-    return -1;
-  }
-  return 0;
-}
-
 namespace {
 
 // The batch: plans of every item, prologues, the frame-loop launches grouped
@@ -1048,6 +1014,8 @@ void run_batch(const nf3d_run_item* items, int n, hipStream_t s, bool timed) {
           B.max_frames = std::max(B.max_frames, R.nframes);
           evs.push_back(e->timer(timed, R.nframes, s));
         }
+        // the seam pass follows on the same stream: run beside the next
+        // launch it takes issue slots the latency-bound k_run needs more
         check(nf3d::launch_run(plans[i].mode, e0->ep.bt, plans[i].shp, e0->ep.bw, e0->ep.bh, B, s),
               "frame-loop kernel");
         check(nf3d::launch_seam_run(e0->ep.bw, e0->ep.bh, B, s), "seam kernel");
@@ -1078,6 +1046,22 @@ std::vector<std::unique_lock<std::mutex>> lock_batch(const nf3d_run_item* items,
 }
 
 }  // namespace
+
+int nf3d_engine_run_device(nf3d_engine* e, int first, int count, const void* const* src, int src_stride,
+                           void* const* dst, int dst_stride, void* stream, char* err, size_t errlen) {
+  std::lock_guard<std::mutex> lock(e->mu);
+  try {
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : e->stream;
+    check(hipSetDevice(e->device), "hipSetDevice");
+    nf3d_run_item it{e, first, count, src, src_stride, dst, dst_stride};
+    run_batch(&it, 1, s, true);
+  } catch (const Failure& f) {
+    copy_err(err, errlen, f.msg);
+    return -1;
+  }
+  return 0;
+}
+
 
 struct nf3d_graph {
   hipGraph_t graph = nullptr;
@@ -1163,6 +1147,12 @@ void nf3d_graph_destroy(nf3d_graph* g) {
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
   delete g;
+}
+
+int nf3d_dft2d_mfma(const void* in, void* out, int nblocks, int n, int inverse, float scale, void* stream) {
+  return nf3d::launch_dft_mfma(in, out, nblocks, n, inverse, scale, static_cast<hipStream_t>(stream)) == hipSuccess
+             ? 0
+             : -1;
 }
 
 int nf3d_engine_cache_refresh(nf3d_engine* e, int n) {

@@ -127,6 +127,8 @@ struct PatternJob {
 // launch_tile runs k_tile and, for the filtering modes, k_seam.
 hipError_t launch_tile(int mode, int bt, int onchip, int bw, int bh, const TileLaunch& l, hipStream_t s);
 hipError_t launch_pattern(const PatternJob& job, hipStream_t s);
+// DFT-as-GEMM precision form (nf3d_mfma_dft.hip): n x n blocks, n = 32.
+hipError_t launch_dft_mfma(const void* in, void* out, int nb, int n, int inverse, float scale, hipStream_t s);
 // k_run over the batch's jobs; `shp` is a Shp value.  launch_seam_run
 // finishes the seams of their frames (after the k_run, same stream).
 hipError_t launch_run(int mode, int bt, int shp, int bw, int bh, const RunBatch& b, hipStream_t s);
