@@ -134,6 +134,54 @@ void nf3d_graph_destroy(nf3d_graph* graph);
  * work is enqueued on `stream`.  0 on success. */
 int nf3d_dft2d_mfma(const void* in, void* out, int nblocks, int n, int inverse, float scale, void* stream);
 
+/* ------------------------------------------------------------ learned PSD
+ * No reference counterpart.  A small MLP maps a block's log-power spectrum
+ * (n_in = bh * (bw/2+1) values) through two GELU hidden layers to a per-block
+ * noise log-PSD (n_out values); the engine can use it in place of the single
+ * noise pattern of src/helper.cpp:57-132 (per-block pattern in the Wiener
+ * gains).  BF16 matrix-core kernels with FP32 accumulation; FP32 master
+ * weights and Adam state.  Parameters in host order W1 [n_in][h], b1 [h],
+ * W2 [h][h], b2 [h], W3 [h][n_out], b3 [n_out].  Features and batches are
+ * BF16 device arrays, rows of n_in.  0 on success, -1 with `err` set. */
+typedef struct nf3d_mlp nf3d_mlp;
+int nf3d_mlp_create(int n_in, int n_hidden, int n_out, int max_batch, nf3d_mlp** out, char* err, size_t errlen);
+void nf3d_mlp_destroy(nf3d_mlp* m);
+long long nf3d_mlp_param_count(const nf3d_mlp* m);
+void nf3d_mlp_dims(const nf3d_mlp* m, int* out3);  /* n_in, n_hidden, n_out */
+int nf3d_mlp_set_params(nf3d_mlp* m, const float* host, char* err, size_t errlen);  /* resets Adam */
+int nf3d_mlp_get_params(nf3d_mlp* m, float* host, char* err, size_t errlen);
+int nf3d_mlp_get_grads(nf3d_mlp* m, float* host, char* err, size_t errlen);
+/* Feature x_k = (ln(|c_k|^2 + eps) - mu_k) * isd_k of a compact spectrum c. */
+int nf3d_mlp_set_input_norm(nf3d_mlp* m, const float* mu, const float* isd, float eps, char* err, size_t errlen);
+int nf3d_mlp_features(nf3d_mlp* m, const void* spec, int n, void* x_out, void* stream);
+/* FP32 -> BF16 (round to nearest even); |c|^2 of `count` complex values. */
+int nf3d_to_bf16(const float* x, void* out, long long count, void* stream);
+int nf3d_periodogram(const void* spec, float* out, long long count, void* stream);
+/* Inference over n rows: out = log-PSD, or exp(log-PSD) * scale when
+ * exp_out; `work` holds nf3d_mlp_work_bytes(m, n) bytes. */
+long long nf3d_mlp_work_bytes(const nf3d_mlp* m, int n);
+int nf3d_mlp_predict(nf3d_mlp* m, const void* x, int n, float* out, int exp_out, float scale, void* work,
+                     void* stream, char* err, size_t errlen);
+/* Gradients of the Whittle loss mean(y + t e^-y) (y = predicted log-PSD,
+ * t = periodogram targets [n][n_out] FP32) over a batch of 1..max_batch rows;
+ * `loss` (optional) waits for the stream.  train_step adds an Adam update
+ * (beta 0.9 / 0.999, eps 1e-8). */
+int nf3d_mlp_grad(nf3d_mlp* m, const void* x, const float* t, int n, double* loss, void* stream, char* err,
+                  size_t errlen);
+int nf3d_mlp_adam(nf3d_mlp* m, float lr, float beta1, float beta2, float eps, void* stream, char* err,
+                  size_t errlen);
+int nf3d_mlp_train_step(nf3d_mlp* m, const void* x, const float* t, int n, float lr, double* loss, void* stream,
+                        char* err, size_t errlen);
+/* The plane's own analysis (FrameToCover + window + r2c, as the filter
+ * computes it) of one device plane into compact spectra [nb][bh][bw/2+1]. */
+int nf3d_engine_spectra(nf3d_engine* e, const void* src, int src_stride, void* spec_out, void* stream, char* err,
+                        size_t errlen);
+/* Use the model's per-block PSD (times `strength`) as the noise pattern of
+ * every frame (null model: back to the reference's sigma / pattern).  The
+ * model must be n_in = n_out = bh * (bw/2+1) and must not be trained while
+ * an engine uses it.  Frames then go through the per-frame path. */
+int nf3d_engine_set_learned(nf3d_engine* e, nf3d_mlp* m, float strength, char* err, size_t errlen);
+
 /* Replaces FFT3DEngine::CacheRefresh (src/fft3d_engine.h:736-740): 1 if the
  * spectrum of frame n is held in the HBM spectra ring (and marks it recent). */
 int nf3d_engine_cache_refresh(nf3d_engine* e, int n);

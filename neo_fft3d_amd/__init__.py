@@ -68,6 +68,13 @@ def lib() -> ctypes.CDLL:
     """The product library; raises Nf3dError when it is not built."""
     global _lib
     if _lib is None:
+        # One HIP runtime per process: torch ships its own libamdhip64.so.7;
+        # loaded first, it also serves this library (same soname).  Loaded
+        # after the system runtime, torch finds no device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise Nf3dError(f"{LIB_PATH} is missing: build it with `python -m neo_fft3d_amd.build`")
         L = ctypes.CDLL(LIB_PATH)

@@ -29,7 +29,7 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument", f"-I{os.path.join(ROOT, 'include')}",
 ]
 
-KERNEL_UNITS = ["nf3d_kernels.hip", "nf3d_mfma_dft.hip"]
+KERNEL_UNITS = ["nf3d_kernels.hip", "nf3d_mfma_dft.hip", "nf3d_mlp.hip"]
 HOST_UNITS = ["nf3d_engine.cpp", "nf3d_frontend.cpp"]
 
 

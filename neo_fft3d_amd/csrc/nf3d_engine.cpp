@@ -114,6 +114,14 @@ struct nf3d_engine {
   // pass of the one before for its buffer)
   float* d_seam_run = nullptr;
   static constexpr int kRunFrames = nf3d::RUN_MAX_FRAMES;  // frames per k_run launch
+  // learned PSD: model, strength, the frame's spectra, features, per-block
+  // pattern [nb][bh][outwidth] and the model's inference workspace
+  nf3d_mlp* learned = nullptr;
+  float learned_strength = 1.0f;
+  float2* d_lspec = nullptr;
+  void* d_lx = nullptr;
+  float* d_lpat = nullptr;
+  void* d_lwork = nullptr;
 
   ~nf3d_engine() {
     hipFree(d_tab);
@@ -128,6 +136,10 @@ struct nf3d_engine {
     hipFree(d_spec_tmp);
     hipFree(d_psums);
     hipFree(d_choice);
+    hipFree(d_lspec);
+    hipFree(d_lx);
+    hipFree(d_lpat);
+    hipFree(d_lwork);
     for (auto* p : d_stage) hipFree(p);
     hipFree(d_dst);
     for (auto ev : ev_pool) hipEventDestroy(ev);
@@ -662,13 +674,32 @@ struct nf3d_engine {
     launch(nf3d::MODE_SHARPEN, 0, J, c.s, true);
   }
 
+  // Learned PSD of frame n's blocks: the frame's own analysis, features,
+  // exp(model log-PSD) * mult into d_lpat; J reads it as a per-block pattern.
+  void learned_psd(int n, const Call& c, float mult, PlaneJob& J) {
+    Src s0 = source(c, n);
+    PlaneJob F = base_job();
+    F.src = s0.p;
+    F.src_pitch = s0.pitch;
+    F.spec_out = d_lspec;
+    launch(nf3d::MODE_FWD, 0, F, c.s);
+    if (nf3d_mlp_features(learned, d_lspec, nb, d_lx, c.s)) fail("learned PSD: feature kernel");
+    char msg[256] = {0};
+    if (nf3d_mlp_predict(learned, d_lx, nb, d_lpat, 1, mult * learned_strength, d_lwork, c.s, msg, sizeof msg))
+      fail(std::string("learned PSD: ") + msg);
+    J.use_pattern = 1;
+    J.pattern_per_block = 1;
+    J.pattern2d = d_lpat;
+    J.pattern3d = d_lpat;
+  }
+
   void run(int n, const Call& c, uint8_t* dst, long long dst_pitch) {
     check(hipSetDevice(device), "hipSetDevice");
     if (ep.pfactor != 0 && ep.pshow) {
       run_pshow(n, c, dst, dst_pitch);
       return;
     }
-    if (ep.pfactor != 0 && !pattern_set) estimate_pattern(c);
+    if (ep.pfactor != 0 && !pattern_set && !learned) estimate_pattern(c);
     int btcur = ep.bt;
     if ((ep.bt / 2 > n) || (ep.bt - 1) / 2 > (frames - 1 - n)) btcur = 1;
     PlaneJob J = base_job();
@@ -691,9 +722,11 @@ struct nf3d_engine {
         J.src_pitch = s0.pitch;
         // keep the spectrum for the neighbouring 3D windows (results-neutral)
         if (ring_slots && ring_lookup(n) < 0) J.spec_out = slot_ptr(ring_claim(n, n, n));
+        if (learned) learned_psd(n, c, 1.0f, J);
         launch(nf3d::MODE_W2D, 0, J, c.s, true);
       } else {
-        if (J.use_pattern) ensure_pattern3d(btcur, c.s);
+        if (J.use_pattern && !learned) ensure_pattern3d(btcur, c.s);
+        if (learned) learned_psd(n, c, (float)btcur, J);
         const int from = -btcur / 2, to = (btcur - 1) / 2;
         for (int i = from; i < to; i++) {  // older frames: from the ring, else transformed now
           int k = n + i;
@@ -722,6 +755,7 @@ struct nf3d_engine {
         launch(nf3d::MODE_W3D, btcur, J, c.s, true, onchip);
       }
     } else {
+      if (learned && ep.bt == 0) learned_psd(n, c, 1.0f, J);
       Src s0 = source(c, n);
       J.src = s0.p;
       J.src_pitch = s0.pitch;
@@ -738,7 +772,7 @@ struct nf3d_engine {
     const int lo = ep.bt > 1 ? std::max(n - ep.bt / 2, 0) : n;
     const int hi = ep.bt > 1 ? std::min(n + (ep.bt - 1) / 2, frames - 1) : n;
     for (int i = lo; i <= hi; i++) win[k++] = nf3d_frame_ref{i, src[i], (int)src_pitch};
-    if (ep.pfactor != 0 && !pattern_set) {
+    if (ep.pfactor != 0 && !pattern_set && !learned) {
       const int pf = std::max(0, std::min(ep.pframe, frames - 1));
       if (pf < lo || pf > hi) win[k++] = nf3d_frame_ref{pf, src[pf], (int)src_pitch};
     }
@@ -803,7 +837,7 @@ struct nf3d_engine {
                       (long long)(tc + Hc) * dst_pitch < (1LL << 31) && 4LL * Hc * Wc < (1LL << 31) &&
                       dst_pitch % pix == 0;
     const bool pshow = ep.pfactor != 0 && ep.pshow;
-    P.run = P.a <= P.b && fits && !pshow && !(P.mode == nf3d::MODE_SHARPEN && P.shp == nf3d::SHP_NONE);
+    P.run = P.a <= P.b && fits && !pshow && !learned && !(P.mode == nf3d::MODE_SHARPEN && P.shp == nf3d::SHP_NONE);
     if (!P.run) return P;
     // launches of equal length (within one frame), two halves of seam
     // buffers in turn
@@ -842,7 +876,7 @@ struct nf3d_engine {
     const int stop = P.run ? P.a : P.first + P.count;
     for (int n = P.first; n < stop; n++) run_one(n, P.src, P.src_pitch, P.dst, P.dst_pitch, s);
     if (!P.run) return;
-    if (ep.pfactor != 0 && !pattern_set) {
+    if (ep.pfactor != 0 && !pattern_set && !learned) {
       const int pf = std::max(0, std::min(ep.pframe, frames - 1));
       nf3d_frame_ref r{pf, P.src[pf], (int)P.src_pitch};
       estimate_pattern(Call{&r, 1, true, s});
@@ -1153,6 +1187,55 @@ int nf3d_dft2d_mfma(const void* in, void* out, int nblocks, int n, int inverse, 
   return nf3d::launch_dft_mfma(in, out, nblocks, n, inverse, scale, static_cast<hipStream_t>(stream)) == hipSuccess
              ? 0
              : -1;
+}
+
+int nf3d_engine_spectra(nf3d_engine* e, const void* src, int src_stride, void* spec_out, void* stream, char* err,
+                        size_t errlen) {
+  std::lock_guard<std::mutex> lock(e->mu);
+  try {
+    if (!src || !spec_out) fail("null plane or spectrum");
+    check(hipSetDevice(e->device), "hipSetDevice");
+    PlaneJob F = e->base_job();
+    F.src = static_cast<const uint8_t*>(src);
+    F.src_pitch = src_stride;
+    F.spec_out = static_cast<float2*>(spec_out);
+    e->launch(nf3d::MODE_FWD, 0, F, stream ? static_cast<hipStream_t>(stream) : e->stream);
+  } catch (const Failure& f) {
+    copy_err(err, errlen, f.msg);
+    return -1;
+  }
+  return 0;
+}
+
+int nf3d_engine_set_learned(nf3d_engine* e, nf3d_mlp* m, float strength, char* err, size_t errlen) {
+  std::lock_guard<std::mutex> lock(e->mu);
+  try {
+    check(hipSetDevice(e->device), "hipSetDevice");
+    if (!m) {
+      e->learned = nullptr;
+      return 0;
+    }
+    int d[3];
+    nf3d_mlp_dims(m, d);
+    const int want = e->ep.bh * e->outwidth;
+    if (d[0] != want || d[2] != want)
+      fail("learned PSD: the model must have " + std::to_string(want) + " inputs and outputs");
+    if (!(strength > 0.0f)) fail("learned PSD: strength must be positive");
+    if (!e->d_lspec) {
+      e->d_lspec = dev_alloc<float2>(e->spec_elems);
+      e->d_lpat = dev_alloc<float>(e->spec_elems);
+      e->d_lx = dev_alloc<uint16_t>(e->spec_elems);
+    }
+    hipFree(e->d_lwork);
+    e->d_lwork = nullptr;
+    e->d_lwork = dev_alloc<uint8_t>((size_t)nf3d_mlp_work_bytes(m, e->nb));
+    e->learned = m;
+    e->learned_strength = strength;
+  } catch (const Failure& f) {
+    copy_err(err, errlen, f.msg);
+    return -1;
+  }
+  return 0;
 }
 
 int nf3d_engine_cache_refresh(nf3d_engine* e, int n) {

@@ -57,6 +57,7 @@ struct PlaneJob {
   float degrid;
   float sharpen, smin_n, smax_n, dehalo, ht2n, kratio2;
   int use_pattern, use_degrid;
+  int pattern_per_block;  // pattern2d / pattern3d hold one [bh][outwidth] pattern per block (learned PSD)
   // spectra, layout [block][bh][outwidth] float2 (compact)
   float2* spec_out;          // where this launch's forward spectra are stored (or null)
   const float2* spec_in[5];  // Wiener-3D window, oldest first (on-chip newest: unused)

@@ -1,0 +1,672 @@
+// Learned noise PSD: a small MLP over a block's log-power spectrum
+// (n_in = bh * (bw/2+1) inputs -> hidden -> hidden -> n_out log-PSD outputs,
+// GELU), trained and run on the matrix cores.
+//
+// Numerics: BF16 operands (weights, activations, gradients) with FP32
+// accumulation (v_mfma_f32_16x16x32_bf16); FP32 master weights, biases,
+// pre-activations and Adam moments.
+//
+// Kernels:
+//   k_gemm<TA, TB, EPI>   C(m, n) = sum_k A(m, k) B(k, n) over a 64x64 tile
+//                         per workgroup (4 waves, 32x32 each = 2x2 MFMA tiles),
+//                         K in steps of 64 staged through LDS, next step's
+//                         operands loaded into registers during the MFMAs;
+//                         epilogues: bias + GELU, bias, bias + exp (PSD),
+//                         GELU backward, plain FP32 (weight gradients)
+//   k_features            log-power features of compact block spectra
+//   k_whittle             Whittle loss y + t e^-y and its gradient
+//   k_colsum              bias gradients (column sums, two passes)
+//   k_adam                Adam update of the FP32 master and its BF16 copy
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/nf3d.h"
+
+namespace nf3d_mlp_k {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int TM = 64, TN = 64, TK = 64, NT = 256;
+constexpr int LDK = TK + 8;  // LDS row: 64 k-values + 8 pad (144 bytes, 16-byte aligned)
+
+enum Epi : int {
+  EPI_F32 = 0,        // C = acc
+  EPI_BIAS_GELU = 1,  // z = acc + bias -> C = z (if C), H = gelu(z)
+  EPI_BIAS = 2,       // C = acc + bias
+  EPI_BIAS_EXP = 3,   // C = exp(acc + bias) * scale
+  EPI_GELU_BWD = 4,   // g = acc * gelu'(Zp) -> H = g, C = g (if C)
+};
+
+struct GemmArgs {
+  const __bf16* A;  // A(m, k) = TA ? A[k * lda + m] : A[m * lda + k]
+  const __bf16* B;  // B(k, n) = TB ? B[n * ldb + k] : B[k * ldb + n]
+  long long lda, ldb;
+  int M, N, K;
+  const float* bias;  // [N]
+  float* C;
+  long long ldc;
+  __bf16* H;
+  long long ldh;
+  const float* Zp;  // pre-activation of the GELU being differentiated
+  long long ldz;
+  float scale;
+};
+
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float z) {
+  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+// One operand tile (64 rows of the outer index x 64 k) as 512 chunks of 8
+// contiguous elements; each lane loads two.  KC: k is the contiguous index
+// in memory (row-major A, transposed B), else the outer index is.
+template <bool KC>
+struct TileLoad {
+  uint4 v[2];
+  __device__ void load(const __bf16* P, long long ld, int o0, int olim, int k0, int klim, int t) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int c = t + h * NT;
+      int o, k;
+      if constexpr (KC) {
+        o = c >> 3;
+        k = (c & 7) * 8;
+      } else {
+        k = c >> 3;
+        o = (c & 7) * 8;
+      }
+      const int go = o0 + o, gk = k0 + k;
+      v[h] = make_uint4(0, 0, 0, 0);
+      if (go < olim && gk < klim)
+        v[h] = *reinterpret_cast<const uint4*>(KC ? P + (long long)go * ld + gk : P + (long long)gk * ld + go);
+    }
+  }
+  __device__ void store(__bf16* S, int t) const {
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int c = t + h * NT;
+      if constexpr (KC) {
+        *reinterpret_cast<uint4*>(S + (c >> 3) * LDK + (c & 7) * 8) = v[h];
+      } else {
+        const int k = c >> 3, o = (c & 7) * 8;
+        __bf16 e[8];
+        memcpy(e, &v[h], 16);
+#pragma unroll
+        for (int i = 0; i < 8; i++) S[(o + i) * LDK + k] = e[i];
+      }
+    }
+  }
+};
+
+template <bool TA, bool TB, int EPI>
+__global__ void __launch_bounds__(NT) k_gemm(GemmArgs g) {
+  __shared__ __attribute__((aligned(16))) __bf16 As[TM * LDK];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[TN * LDK];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+  const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
+  TileLoad<!TA> la;
+  TileLoad<TB> lb;
+  la.load(g.A, g.lda, m0, g.M, 0, g.K, t);
+  lb.load(g.B, g.ldb, n0, g.N, 0, g.K, t);
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  const int r16 = lane & 15, kg = (lane >> 4) * 8;
+  for (int k0 = 0; k0 < g.K; k0 += TK) {
+    __syncthreads();  // the previous step's MFMAs are done with the tiles
+    la.store(As, t);
+    lb.store(Bs, t);
+    __syncthreads();
+    if (k0 + TK < g.K) {
+      la.load(g.A, g.lda, m0, g.M, k0 + TK, g.K, t);
+      lb.load(g.B, g.ldb, n0, g.N, k0 + TK, g.K, t);
+    }
+#pragma unroll
+    for (int kk = 0; kk < TK; kk += 32) {
+      bf16x8 a[2], b[2];
+#pragma unroll
+      for (int i = 0; i < 2; i++) a[i] = *reinterpret_cast<const bf16x8*>(As + (wm + i * 16 + r16) * LDK + kk + kg);
+#pragma unroll
+      for (int j = 0; j < 2; j++) b[j] = *reinterpret_cast<const bf16x8*>(Bs + (wn + j * 16 + r16) * LDK + kk + kg);
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+  }
+  // epilogue: lane holds rows 4*(lane/16) + r of each 16x16 tile, column lane % 16
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    const int n = n0 + wn + j * 16 + r16;
+    if (n >= g.N) continue;
+    const float bias = (EPI == EPI_BIAS_GELU || EPI == EPI_BIAS || EPI == EPI_BIAS_EXP) ? g.bias[n] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
+        if (m >= g.M) continue;
+        const float v = acc[i][j][r];
+        if constexpr (EPI == EPI_F32) {
+          g.C[(long long)m * g.ldc + n] = v;
+        } else if constexpr (EPI == EPI_BIAS_GELU) {
+          const float z = v + bias;
+          if (g.C) g.C[(long long)m * g.ldc + n] = z;
+          g.H[(long long)m * g.ldh + n] = (__bf16)gelu(z);
+        } else if constexpr (EPI == EPI_BIAS) {
+          g.C[(long long)m * g.ldc + n] = v + bias;
+        } else if constexpr (EPI == EPI_BIAS_EXP) {
+          const float z = fminf(v + bias, 80.0f);
+          g.C[(long long)m * g.ldc + n] = expf(z) * g.scale;
+        } else {
+          const float d = v * gelu_grad(g.Zp[(long long)m * g.ldz + n]);
+          g.H[(long long)m * g.ldh + n] = (__bf16)d;
+          if (g.C) g.C[(long long)m * g.ldc + n] = d;
+        }
+      }
+  }
+}
+
+// x = (ln(|c|^2 + eps) - mu[k]) * isd[k] over n blocks of d compact elements.
+__global__ void k_features(const float2* spec, __bf16* X, long long total, int d, const float* mu, const float* isd,
+                           float eps) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int k = (int)(i % d);
+  const float2 c = spec[i];
+  const float p = c.x * c.x + c.y * c.y + eps;
+  X[i] = (__bf16)((logf(p) - mu[k]) * isd[k]);
+}
+
+__global__ void k_to_bf16(const float* x, __bf16* y, long long total) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < total) y[i] = (__bf16)x[i];
+}
+
+__global__ void k_periodogram(const float2* spec, float* T, long long total) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < total) {
+    const float2 c = spec[i];
+    T[i] = c.x * c.x + c.y * c.y;
+  }
+}
+
+// Whittle negative log-likelihood of periodogram values t under the
+// predicted log-PSD y: mean of y + t e^-y; gradient (1 - t e^-y) / count.
+// Per-workgroup partial sums of the loss (the host adds them).
+__global__ void __launch_bounds__(256) k_whittle(const float* Y, const float* T, __bf16* dY, float* dYf,
+                                                 long long total, float inv_count, float* partial) {
+  __shared__ float red[256];
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  float l = 0.0f;
+  if (i < total) {
+    const float y = fminf(fmaxf(Y[i], -80.0f), 80.0f);
+    const float te = T[i] * expf(-y);
+    l = y + te;
+    const float gr = (1.0f - te) * inv_count;
+    dY[i] = (__bf16)gr;
+    dYf[i] = gr;
+  }
+  red[threadIdx.x] = l;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// Column sums of X [rows][cols]: pass 1 writes part[slice][col] over row
+// slices, pass 2 adds the slices.
+constexpr int CS_SLICES = 16;
+__global__ void __launch_bounds__(256) k_colsum1(const float* X, int rows, int cols, float* part) {
+  __shared__ float red[4][64];
+  const int c = blockIdx.x * 64 + (threadIdx.x & 63), q = threadIdx.x >> 6;
+  const int per = (rows + CS_SLICES - 1) / CS_SLICES;
+  const int r0 = blockIdx.y * per, r1 = min(rows, r0 + per);
+  float s = 0.0f;
+  if (c < cols)
+    for (int r = r0 + q; r < r1; r += 4) s += X[(long long)r * cols + c];
+  red[q][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (q == 0 && c < cols)
+    part[blockIdx.y * cols + c] = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+}
+__global__ void k_colsum2(const float* part, int cols, float* out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cols) return;
+  float s = 0.0f;
+  for (int q = 0; q < CS_SLICES; q++) s += part[q * cols + c];
+  out[c] = s;
+}
+
+__global__ void k_adam(float* p, const float* g, float* m, float* v, __bf16* pb, long long n, float lr, float b1,
+                       float b2, float eps, float c1, float c2) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float pi = p[i] - lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+  p[i] = pi;
+  pb[i] = (__bf16)pi;
+}
+
+inline unsigned blocks(long long n, int t) { return (unsigned)((n + t - 1) / t); }
+
+template <bool TA, bool TB, int EPI>
+hipError_t gemm(const GemmArgs& g, hipStream_t s) {
+  dim3 grid((g.N + TN - 1) / TN, (g.M + TM - 1) / TM);
+  hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), grid, dim3(NT), 0, s, g);
+  return hipGetLastError();
+}
+
+GemmArgs args(const __bf16* A, long long lda, const __bf16* B, long long ldb, int M, int N, int K) {
+  GemmArgs g{};
+  g.A = A;
+  g.lda = lda;
+  g.B = B;
+  g.ldb = ldb;
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.scale = 1.0f;
+  return g;
+}
+
+}  // namespace nf3d_mlp_k
+
+using namespace nf3d_mlp_k;
+
+namespace {
+
+struct MlpFail {
+  std::string msg;
+};
+void mcheck(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw MlpFail{std::string(what) + ": " + hipGetErrorString(e)};
+}
+void merr(char* err, size_t len, const std::string& m) {
+  if (err && len) {
+    size_t n = std::min(len - 1, m.size());
+    memcpy(err, m.data(), n);
+    err[n] = 0;
+  }
+}
+template <typename T>
+T* dalloc(size_t count) {
+  void* p = nullptr;
+  mcheck(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)), "hipMalloc");
+  return static_cast<T*>(p);
+}
+
+}  // namespace
+
+struct nf3d_mlp {
+  int n_in = 0, nh = 0, n_out = 0, max_batch = 0, device = 0;
+  size_t o_w1 = 0, o_b1 = 0, o_w2 = 0, o_b2 = 0, o_w3 = 0, o_b3 = 0, np = 0;
+  float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
+  __bf16* pb = nullptr;
+  float *mu = nullptr, *isd = nullptr;
+  float eps = 1e-10f;
+  // training workspace, max_batch rows
+  float *z1 = nullptr, *z2 = nullptr, *y = nullptr, *gf = nullptr, *part = nullptr, *lpart = nullptr;
+  __bf16 *h1 = nullptr, *h2 = nullptr, *d3 = nullptr, *d2 = nullptr, *d1 = nullptr;
+  int step = 0;
+  std::mutex mu_lock;
+
+  ~nf3d_mlp() {
+    for (void* q : {(void*)p, (void*)g, (void*)m, (void*)v, (void*)pb, (void*)mu, (void*)isd, (void*)z1, (void*)z2,
+                    (void*)y, (void*)gf, (void*)part, (void*)lpart, (void*)h1, (void*)h2, (void*)d3, (void*)d2,
+                    (void*)d1})
+      hipFree(q);
+  }
+
+  void init(int ni, int h, int no, int mb) {
+    if (ni <= 0 || h <= 0 || no <= 0 || mb <= 0 || ni % 8 || h % 8 || no % 8)
+      throw MlpFail{"layer widths must be positive multiples of 8"};
+    n_in = ni;
+    nh = h;
+    n_out = no;
+    max_batch = mb;
+    mcheck(hipGetDevice(&device), "hipGetDevice");
+    // every matrix starts at a multiple of 8 elements (16-byte BF16 loads)
+    auto take = [&](size_t n) {
+      size_t o = np;
+      np += (n + 7) / 8 * 8;
+      return o;
+    };
+    o_w1 = take((size_t)ni * h);
+    o_b1 = take(h);
+    o_w2 = take((size_t)h * h);
+    o_b2 = take(h);
+    o_w3 = take((size_t)h * no);
+    o_b3 = take(no);
+    p = dalloc<float>(np);
+    g = dalloc<float>(np);
+    m = dalloc<float>(np);
+    v = dalloc<float>(np);
+    pb = dalloc<__bf16>(np);
+    mcheck(hipMemset(p, 0, np * 4), "hipMemset");
+    mcheck(hipMemset(g, 0, np * 4), "hipMemset");
+    mcheck(hipMemset(m, 0, np * 4), "hipMemset");
+    mcheck(hipMemset(v, 0, np * 4), "hipMemset");
+    mcheck(hipMemset(pb, 0, np * 2), "hipMemset");
+    mu = dalloc<float>(ni);
+    isd = dalloc<float>(ni);
+    std::vector<float> zero(ni, 0.0f), one(ni, 1.0f);
+    mcheck(hipMemcpy(mu, zero.data(), ni * 4, hipMemcpyHostToDevice), "norm");
+    mcheck(hipMemcpy(isd, one.data(), ni * 4, hipMemcpyHostToDevice), "norm");
+    const size_t wide = (size_t)std::max(h, no);
+    z1 = dalloc<float>((size_t)mb * h);
+    z2 = dalloc<float>((size_t)mb * h);
+    y = dalloc<float>((size_t)mb * no);
+    gf = dalloc<float>((size_t)mb * wide);
+    part = dalloc<float>((size_t)CS_SLICES * wide);
+    lpart = dalloc<float>(blocks((long long)mb * no, 256));
+    h1 = dalloc<__bf16>((size_t)mb * h);
+    h2 = dalloc<__bf16>((size_t)mb * h);
+    d3 = dalloc<__bf16>((size_t)mb * no);
+    d2 = dalloc<__bf16>((size_t)mb * h);
+    d1 = dalloc<__bf16>((size_t)mb * h);
+  }
+
+  void params_changed() {
+    hipLaunchKernelGGL(k_to_bf16, dim3(blocks((long long)np, 256)), dim3(256), 0, nullptr, p, pb, (long long)np);
+    mcheck(hipGetLastError(), "k_to_bf16");
+    mcheck(hipDeviceSynchronize(), "params");
+  }
+
+  void colsum(const float* X, int rows, int cols, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_colsum1, dim3((cols + 63) / 64, CS_SLICES), dim3(256), 0, s, X, rows, cols, part);
+    hipLaunchKernelGGL(k_colsum2, dim3(blocks(cols, 256)), dim3(256), 0, s, part, cols, out);
+    mcheck(hipGetLastError(), "k_colsum");
+  }
+
+  // Inference: log-PSD (exp_out = 0) or exp(log-PSD) * scale into out [n][n_out].
+  void predict(const __bf16* X, int n, float* out, int exp_out, float scale, __bf16* w1, __bf16* w2,
+               hipStream_t s) {
+    GemmArgs a = args(X, n_in, pb + o_w1, nh, n, nh, n_in);
+    a.bias = p + o_b1;
+    a.H = w1;
+    a.ldh = nh;
+    mcheck(gemm<false, false, EPI_BIAS_GELU>(a, s), "layer 1");
+    a = args(w1, nh, pb + o_w2, nh, n, nh, nh);
+    a.bias = p + o_b2;
+    a.H = w2;
+    a.ldh = nh;
+    mcheck(gemm<false, false, EPI_BIAS_GELU>(a, s), "layer 2");
+    a = args(w2, nh, pb + o_w3, n_out, n, n_out, nh);
+    a.bias = p + o_b3;
+    a.C = out;
+    a.ldc = n_out;
+    a.scale = scale;
+    if (exp_out) mcheck(gemm<false, false, EPI_BIAS_EXP>(a, s), "layer 3");
+    else mcheck(gemm<false, false, EPI_BIAS>(a, s), "layer 3");
+  }
+
+  // Forward with the activations kept, Whittle loss, gradients into g.
+  void grad(const __bf16* X, const float* T, int n, hipStream_t s) {
+    GemmArgs a = args(X, n_in, pb + o_w1, nh, n, nh, n_in);
+    a.bias = p + o_b1;
+    a.C = z1;
+    a.ldc = nh;
+    a.H = h1;
+    a.ldh = nh;
+    mcheck(gemm<false, false, EPI_BIAS_GELU>(a, s), "layer 1");
+    a = args(h1, nh, pb + o_w2, nh, n, nh, nh);
+    a.bias = p + o_b2;
+    a.C = z2;
+    a.ldc = nh;
+    a.H = h2;
+    a.ldh = nh;
+    mcheck(gemm<false, false, EPI_BIAS_GELU>(a, s), "layer 2");
+    a = args(h2, nh, pb + o_w3, n_out, n, n_out, nh);
+    a.bias = p + o_b3;
+    a.C = y;
+    a.ldc = n_out;
+    mcheck(gemm<false, false, EPI_BIAS>(a, s), "layer 3");
+    const long long tot = (long long)n * n_out;
+    hipLaunchKernelGGL(k_whittle, dim3(blocks(tot, 256)), dim3(256), 0, s, y, T, d3, gf, tot,
+                       1.0f / (float)tot, lpart);
+    mcheck(hipGetLastError(), "k_whittle");
+    // layer 3: dW3 = H2^T dY, db3 = colsum dY, dZ2 = (dY W3^T) * gelu'(Z2)
+    a = args(h2, nh, d3, n_out, nh, n_out, n);
+    a.C = g + o_w3;
+    a.ldc = n_out;
+    mcheck(gemm<true, false, EPI_F32>(a, s), "dW3");
+    colsum(gf, n, n_out, g + o_b3, s);
+    a = args(d3, n_out, pb + o_w3, n_out, n, nh, n_out);
+    a.H = d2;
+    a.ldh = nh;
+    a.C = gf;
+    a.ldc = nh;
+    a.Zp = z2;
+    a.ldz = nh;
+    mcheck(gemm<false, true, EPI_GELU_BWD>(a, s), "dZ2");
+    // layer 2
+    a = args(h1, nh, d2, nh, nh, nh, n);
+    a.C = g + o_w2;
+    a.ldc = nh;
+    mcheck(gemm<true, false, EPI_F32>(a, s), "dW2");
+    colsum(gf, n, nh, g + o_b2, s);
+    a = args(d2, nh, pb + o_w2, nh, n, nh, nh);
+    a.H = d1;
+    a.ldh = nh;
+    a.C = gf;
+    a.ldc = nh;
+    a.Zp = z1;
+    a.ldz = nh;
+    mcheck(gemm<false, true, EPI_GELU_BWD>(a, s), "dZ1");
+    // layer 1
+    a = args(X, n_in, d1, nh, n_in, nh, n);
+    a.C = g + o_w1;
+    a.ldc = nh;
+    mcheck(gemm<true, false, EPI_F32>(a, s), "dW1");
+    colsum(gf, n, nh, g + o_b1, s);
+  }
+
+  double read_loss(int n, hipStream_t s) {
+    const long long tot = (long long)n * n_out;
+    std::vector<float> h(blocks(tot, 256));
+    mcheck(hipMemcpyAsync(h.data(), lpart, h.size() * 4, hipMemcpyDeviceToHost, s), "loss");
+    mcheck(hipStreamSynchronize(s), "loss");
+    double sum = 0;
+    for (float x : h) sum += x;
+    return sum / (double)tot;
+  }
+
+  void adam(float lr, float b1, float b2, float eps_, hipStream_t s) {
+    step++;
+    const float c1 = (float)(1.0 / (1.0 - std::pow((double)b1, step)));
+    const float c2 = (float)(1.0 / (1.0 - std::pow((double)b2, step)));
+    hipLaunchKernelGGL(k_adam, dim3(blocks((long long)np, 256)), dim3(256), 0, s, p, g, m, v, pb, (long long)np, lr,
+                       b1, b2, eps_, c1, c2);
+    mcheck(hipGetLastError(), "k_adam");
+  }
+};
+
+namespace {
+
+template <class F>
+int guarded(char* err, size_t errlen, F&& f) {
+  try {
+    f();
+  } catch (const MlpFail& e) {
+    merr(err, errlen, e.msg);
+    return -1;
+  } catch (const std::bad_alloc&) {
+    merr(err, errlen, "out of host memory");
+    return -1;
+  }
+  return 0;
+}
+
+hipStream_t st(void* s) { return static_cast<hipStream_t>(s); }
+
+}  // namespace
+
+// ---------------------------------------------------------------- C ABI
+extern "C" {
+
+int nf3d_mlp_create(int n_in, int n_hidden, int n_out, int max_batch, nf3d_mlp** out, char* err, size_t errlen) {
+  if (!out) return -1;
+  auto* m = new nf3d_mlp();
+  int rc = guarded(err, errlen, [&] { m->init(n_in, n_hidden, n_out, max_batch); });
+  if (rc) {
+    delete m;
+    return rc;
+  }
+  *out = m;
+  return 0;
+}
+
+void nf3d_mlp_destroy(nf3d_mlp* m) { delete m; }
+
+long long nf3d_mlp_param_count(const nf3d_mlp* m) {
+  return (long long)m->n_in * m->nh + m->nh + (long long)m->nh * m->nh + m->nh + (long long)m->nh * m->n_out +
+         m->n_out;
+}
+
+void nf3d_mlp_dims(const nf3d_mlp* m, int* o) {
+  o[0] = m->n_in;
+  o[1] = m->nh;
+  o[2] = m->n_out;
+}
+
+// Host parameters in the order W1 [n_in][h], b1, W2 [h][h], b2, W3 [h][n_out], b3.
+static void pieces(const nf3d_mlp* m, size_t off[6], size_t len[6]) {
+  off[0] = m->o_w1, len[0] = (size_t)m->n_in * m->nh;
+  off[1] = m->o_b1, len[1] = m->nh;
+  off[2] = m->o_w2, len[2] = (size_t)m->nh * m->nh;
+  off[3] = m->o_b2, len[3] = m->nh;
+  off[4] = m->o_w3, len[4] = (size_t)m->nh * m->n_out;
+  off[5] = m->o_b3, len[5] = m->n_out;
+}
+
+int nf3d_mlp_set_params(nf3d_mlp* m, const float* host, char* err, size_t errlen) {
+  std::lock_guard<std::mutex> lock(m->mu_lock);
+  return guarded(err, errlen, [&] {
+    mcheck(hipSetDevice(m->device), "hipSetDevice");
+    size_t off[6], len[6], at = 0;
+    pieces(m, off, len);
+    for (int i = 0; i < 6; i++) {
+      mcheck(hipMemcpy(m->p + off[i], host + at, len[i] * 4, hipMemcpyHostToDevice), "set_params");
+      at += len[i];
+    }
+    mcheck(hipMemset(m->m, 0, m->np * 4), "hipMemset");
+    mcheck(hipMemset(m->v, 0, m->np * 4), "hipMemset");
+    m->step = 0;
+    m->params_changed();
+  });
+}
+
+static int copy_out(nf3d_mlp* m, const float* dev, float* host, char* err, size_t errlen) {
+  std::lock_guard<std::mutex> lock(m->mu_lock);
+  return guarded(err, errlen, [&] {
+    mcheck(hipSetDevice(m->device), "hipSetDevice");
+    mcheck(hipDeviceSynchronize(), "sync");
+    size_t off[6], len[6], at = 0;
+    pieces(m, off, len);
+    for (int i = 0; i < 6; i++) {
+      mcheck(hipMemcpy(host + at, dev + off[i], len[i] * 4, hipMemcpyDeviceToHost), "copy");
+      at += len[i];
+    }
+  });
+}
+
+int nf3d_mlp_get_params(nf3d_mlp* m, float* host, char* err, size_t errlen) {
+  return copy_out(m, m->p, host, err, errlen);
+}
+
+int nf3d_mlp_get_grads(nf3d_mlp* m, float* host, char* err, size_t errlen) {
+  return copy_out(m, m->g, host, err, errlen);
+}
+
+int nf3d_mlp_set_input_norm(nf3d_mlp* m, const float* mu, const float* isd, float eps, char* err, size_t errlen) {
+  std::lock_guard<std::mutex> lock(m->mu_lock);
+  return guarded(err, errlen, [&] {
+    mcheck(hipSetDevice(m->device), "hipSetDevice");
+    mcheck(hipMemcpy(m->mu, mu, m->n_in * 4, hipMemcpyHostToDevice), "norm");
+    mcheck(hipMemcpy(m->isd, isd, m->n_in * 4, hipMemcpyHostToDevice), "norm");
+    m->eps = eps;
+  });
+}
+
+int nf3d_mlp_features(nf3d_mlp* m, const void* spec, int n, void* x_out, void* stream) {
+  if (n < 0 || (n > 0 && (!spec || !x_out))) return -1;
+  const long long tot = (long long)n * m->n_in;
+  if (!tot) return 0;
+  hipLaunchKernelGGL(k_features, dim3(blocks(tot, 256)), dim3(256), 0, st(stream), static_cast<const float2*>(spec),
+                     static_cast<__bf16*>(x_out), tot, m->n_in, m->mu, m->isd, m->eps);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int nf3d_to_bf16(const float* x, void* out, long long count, void* stream) {
+  if (count < 0) return -1;
+  if (!count) return 0;
+  hipLaunchKernelGGL(k_to_bf16, dim3(blocks(count, 256)), dim3(256), 0, st(stream), x, static_cast<__bf16*>(out),
+                     count);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int nf3d_periodogram(const void* spec, float* out, long long count, void* stream) {
+  if (count < 0) return -1;
+  if (!count) return 0;
+  hipLaunchKernelGGL(k_periodogram, dim3(blocks(count, 256)), dim3(256), 0, st(stream),
+                     static_cast<const float2*>(spec), out, count);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+long long nf3d_mlp_work_bytes(const nf3d_mlp* m, int n) { return 2LL * n * m->nh * 2; }
+
+int nf3d_mlp_predict(nf3d_mlp* m, const void* x, int n, float* out, int exp_out, float scale, void* work,
+                     void* stream, char* err, size_t errlen) {
+  return guarded(err, errlen, [&] {
+    if (n < 0 || (n > 0 && (!x || !out || !work))) throw MlpFail{"bad arguments"};
+    if (!n) return;
+    __bf16* w1 = static_cast<__bf16*>(work);
+    m->predict(static_cast<const __bf16*>(x), n, out, exp_out, scale, w1, w1 + (size_t)n * m->nh, st(stream));
+  });
+}
+
+int nf3d_mlp_grad(nf3d_mlp* m, const void* x, const float* t, int n, double* loss, void* stream, char* err,
+                  size_t errlen) {
+  std::lock_guard<std::mutex> lock(m->mu_lock);
+  return guarded(err, errlen, [&] {
+    if (n <= 0 || n > m->max_batch || !x || !t) throw MlpFail{"batch size outside 1..max_batch"};
+    m->grad(static_cast<const __bf16*>(x), t, n, st(stream));
+    if (loss) *loss = m->read_loss(n, st(stream));
+  });
+}
+
+int nf3d_mlp_adam(nf3d_mlp* m, float lr, float beta1, float beta2, float eps, void* stream, char* err,
+                  size_t errlen) {
+  std::lock_guard<std::mutex> lock(m->mu_lock);
+  return guarded(err, errlen, [&] { m->adam(lr, beta1, beta2, eps, st(stream)); });
+}
+
+int nf3d_mlp_train_step(nf3d_mlp* m, const void* x, const float* t, int n, float lr, double* loss, void* stream,
+                        char* err, size_t errlen) {
+  std::lock_guard<std::mutex> lock(m->mu_lock);
+  return guarded(err, errlen, [&] {
+    if (n <= 0 || n > m->max_batch || !x || !t) throw MlpFail{"batch size outside 1..max_batch"};
+    m->grad(static_cast<const __bf16*>(x), t, n, st(stream));
+    if (loss) *loss = m->read_loss(n, st(stream));
+    m->adam(lr, 0.9f, 0.999f, 1e-8f, st(stream));
+  });
+}
+
+}  // extern "C"

@@ -94,11 +94,13 @@ struct Tab {
   float wdh;    // dehalo window
 };
 
+// `poff`: the block's element offset when the pattern is per block (learned
+// PSD), else 0.
 template <bool PAT3D>
-__device__ __forceinline__ Tab load_tab(const PlaneJob& J, int k, bool sharp) {
+__device__ __forceinline__ Tab load_tab(const PlaneJob& J, int k, bool sharp, size_t poff) {
   Tab t;
   t.g = J.use_degrid ? J.grid[k] : make_float2(0.0f, 0.0f);
-  t.sig = J.use_pattern ? (PAT3D ? J.pattern3d[k] : J.pattern2d[k]) : J.sigma2;
+  t.sig = J.use_pattern ? (PAT3D ? J.pattern3d[poff + k] : J.pattern2d[poff + k]) : J.sigma2;
   t.wsh = sharp ? J.wsharpen[k] : 0.0f;
   t.wdh = sharp ? J.wdehalo[k] : 0.0f;
   return t;
@@ -575,7 +577,7 @@ __global__ void __launch_bounds__(NT) k_tile(TileLaunch L_arg) {
     // buffer's DC, code_impl_C.h:64); lane 0 of each block computes both
     if (col_on && cc == 0) {
       const int b = cb;
-      const Tab T0 = load_tab<MODE == MODE_W3D>(J, 0, do_sharp);
+      const Tab T0 = load_tab<MODE == MODE_W3D>(J, 0, do_sharp, J.pattern_per_block ? colo - cc : 0);
       float gf = 0.0f, dcf;
       if constexpr (MODE == MODE_W3D) {
         float2 v[5];
@@ -602,6 +604,7 @@ __global__ void __launch_bounds__(NT) k_tile(TileLaunch L_arg) {
     // ---- 4b. spectral pass down the column; loads issued 4 rows at a time
     if (col_on) {
       const float gf = s_gf[cb], gfs = s_gfs[cb];
+      const size_t poff = J.pattern_per_block ? colo - cc : 0;
       static_for<0, BH / 4>([&](auto ic) {
         constexpr int i0 = decltype(ic)::value * 4;
         float2 v[4][5];
@@ -619,7 +622,7 @@ __global__ void __launch_bounds__(NT) k_tile(TileLaunch L_arg) {
             v[u][0] = col[i];
           }
           if constexpr (MODE == MODE_KALMAN) ks[u] = KalState{J.kal_last[o], J.kal_covar[o], J.kal_cproc[o]};
-          tb[u] = load_tab<MODE == MODE_W3D>(J, i * COLS + cc, do_sharp);
+          tb[u] = load_tab<MODE == MODE_W3D>(J, i * COLS + cc, do_sharp, poff);
         });
         static_for<0, 4>([&](auto uu) {
           constexpr int u = decltype(uu)::value, i = i0 + u;

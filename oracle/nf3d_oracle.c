@@ -194,6 +194,7 @@ typedef struct {
   double *pattern2d, *pattern3d;      /* [bh*outwidth] */
   int pattern_set, pattern3d_init;
   int pshow_flat;                      /* pshow has set the windows to 1 */
+  double* bpsd;                        /* [nb][bh*outwidth] per-block noise PSD of the next frame, or NULL */
   cplx* grid;                          /* [bh*outwidth] */
   float sigmaSquaredNoiseNormed2D, sigmaSquaredSharpenMinNormed, sigmaSquaredSharpenMaxNormed, ht2n;
   /* Kalman state */
@@ -450,7 +451,7 @@ static void apply2d(oengine* e, cplx* X, float sig2f, int pattern) {
       double g0 = gf * e->grid[k].re, g1 = gf * e->grid[k].im;
       double cr = blk[k].re - g0, ci = blk[k].im - g1;
       double psd = cr * cr + ci * ci + (double)1e-15f;
-      double sig2 = pattern ? e->pattern2d[k] : (double)sig2f;
+      double sig2 = pattern ? (e->bpsd ? e->bpsd[(size_t)b * bh * ow + k] : e->pattern2d[k]) : (double)sig2f;
       double f = (psd - sig2) / psd;
       f = f > lowlimit ? f : lowlimit;
       if (!pattern) {
@@ -511,7 +512,7 @@ static void apply3d(oengine* e, cplx* const* in, int btc, int from, cplx* out, f
           fi += v.re * tw[q][t].im + v.im * tw[q][t].re;
         }
         if (q == 0) { fr -= g0; fi -= g1; }
-        double sig2 = pattern ? e->pattern3d[k] : (double)sig2f;
+        double sig2 = pattern ? (e->bpsd ? e->bpsd[off + k] * (double)(float)btc : e->pattern3d[k]) : (double)sig2f;
         double wr, wi;
         wiener(fr, fi, sig2, lowlimit, &wr, &wi);
         sr += wr; si += wi;
@@ -530,7 +531,10 @@ static void kalman(oengine* e, const cplx* cur, int pattern) {
     for (int k = 0; k < bh * ow; k++) {
       size_t o = (size_t)b * bh * ow + k;
       double sigma;
-      if (pattern) { sigma = e->pattern2d[k]; if (sigma < (double)1e-15f) sigma = (double)1e-15f; }
+      if (pattern) {
+        sigma = e->bpsd ? e->bpsd[o] : e->pattern2d[k];
+        if (sigma < (double)1e-15f) sigma = (double)1e-15f;
+      }
       else sigma = (double)e->sigmaSquaredNoiseNormed2D;
       cplx c = cur[o], p = e->outLast[o];
       double dr = c.re - p.re, di = c.im - p.im;
@@ -646,7 +650,7 @@ static const char* engine_init(oengine* e, const oparams* p, const oformat* f, i
 static void engine_free(oengine* e) {
   free(e->wanxl); free(e->wanxr); free(e->wanyl); free(e->wanyr);
   free(e->wsynxl); free(e->wsynxr); free(e->wsynyl); free(e->wsynyr);
-  free(e->wsharpen); free(e->wdehalo); free(e->pwin); free(e->pattern2d); free(e->pattern3d);
+  free(e->wsharpen); free(e->wdehalo); free(e->pwin); free(e->pattern2d); free(e->pattern3d); free(e->bpsd);
   free(e->grid); free(e->outLast); free(e->covar); free(e->covarProcess);
   if (e->spec) for (int i = 0; i < e->frames; i++) free(e->spec[i]);
   free(e->spec); free(e->blk); free(e->tmp);
@@ -682,10 +686,10 @@ static void engine_frame(oclip* c, int pl, int n, uint8_t* dst, int dstride) {
   size_t ssz = (size_t)e->nb * bh * ow;
   const uint8_t* src = c->planes[(size_t)n * c->f.nplanes + pl];
   int sstride = c->strides[pl];
-  int pattern = e->ep.pfactor != 0;
+  int pattern = e->ep.pfactor != 0 || e->bpsd;
   for (int k = 0; k < n - 2 && k < e->frames; k++)
     if (e->spec[k] && k != e->ep.pframe) { free(e->spec[k]); e->spec[k] = NULL; }
-  if (pattern && !e->pattern_set && !e->ep.pshow) {
+  if (pattern && !e->bpsd && !e->pattern_set && !e->ep.pshow) {
     /* src/fft3d_engine.h:485-504 */
     int pf = e->ep.pframe;
     const cplx* ps = spectrum_of(c, pl, pf);
@@ -695,7 +699,7 @@ static void engine_frame(oclip* c, int pl, int n, uint8_t* dst, int dstride) {
     set_pattern(e, ps, px, py);
     e->pattern_set = 1;
   }
-  if (pattern && e->ep.pshow) {
+  if (pattern && !e->bpsd && e->ep.pshow) {
     /* Pattern display, src/fft3d_engine.h:505-565: the pattern block of
      * this frame (searched unless px/py are given) reconstructed alone with
      * analysis and synthesis windows set to 1 -- permanently, so calls after
@@ -731,7 +735,7 @@ static void engine_frame(oclip* c, int pl, int n, uint8_t* dst, int dstride) {
       apply2d(e, out, sig2f, pattern);
       if (pattern) sharpen_pass(e, out);
     } else {
-      if (!e->pattern3d_init) {
+      if (!e->pattern3d_init && !e->bpsd) {
         for (int k = 0; k < bh * ow; k++) e->pattern3d[k] = e->pattern2d[k] * (double)(float)btcur;
         e->pattern3d_init = 1;
       }
@@ -894,6 +898,21 @@ void nf3d_oracle_free(void* h) {
   for (int i = 0; i < 4; i++) if (c->has_eng[i]) engine_free(&c->eng[i]);
   free(c->planes);
   free(c);
+}
+
+/* Per-block noise PSD ([nb][bh*outwidth], float) used in place of sigma and
+ * the pattern by this plane's following frames (the learned-PSD form of the
+ * library, whose model supplies exp(log-PSD) per block; 3D gains use it times
+ * btcur); NULL switches back.  No reference counterpart. */
+int nf3d_oracle_set_block_psd(void* h, int plane, const float* psd) {
+  oclip* c = (oclip*)h;
+  if (!c->has_eng[plane]) return -1;
+  oengine* e = &c->eng[plane];
+  size_t n = (size_t)e->nb * e->ep.bh * e->outwidth;
+  if (!psd) { free(e->bpsd); e->bpsd = NULL; return 0; }
+  if (!e->bpsd) e->bpsd = xcalloc(n, sizeof(double));
+  for (size_t i = 0; i < n; i++) e->bpsd[i] = (double)psd[i];
+  return 0;
 }
 
 /* Per-plane geometry for tests: nox, noy, outwidth, mirw, mirh. */
