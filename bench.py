@@ -16,6 +16,12 @@ frame-loop launches through nf3d_run_device_batch).  Prints one JSON line:
                 oracle/_ref, AVX-512 kernels, opt=4) on a bounded sample of the
                 same workload on this host's cores
   psnr_db       denoised vs clean (noise-free render of the same seed)
+  precision_form  the DFT-as-GEMM form (FP16 split operands on the matrix
+                cores) on 3840x2160 32/16 blocks: rate and error vs FP64
+  learned_psd   the learned-PSD form on RGBPS float planes, bt=3, 48/16 with
+                coloured, spatially varying noise: network trained on the GPU,
+                PSNR of plain (best global sigma) vs learned, frames/s of both
+                at 1920x1080, MLP inference rate
 
 ``--gpus N`` with N > 1 exits: this build runs on one GPU.
 """
@@ -215,6 +221,125 @@ def precision_form(args):
             "round_trip_max_err_10bit": rt_err}
 
 
+def learned_form(args):
+    """Learned-PSD form on RGBPS float planes, bt=3, 48/16 (coloured noise with
+    a Perlin sigma field): the network trained from scratch on the GPU by the
+    library's BF16 matrix-core kernels; PSNR of plain (best global sigma) and
+    learned (best strength), both tuned on one held-out clip and scored on two
+    others; frames/s of both at 1920x1080 on a resident clip."""
+    import numpy as np
+    import torch
+
+    from neo_fft3d_amd import Engine, default_params, make_format, run_device_batch
+    from neo_fft3d_amd import learned as L
+    t0 = time.time()
+    mlp, data, losses = L.train_model(clips=48, frames=2, width=384, height=384, steps=1500)
+    t_train = time.time() - t0
+    fmt = make_format("rgb", 32, True)
+
+    def engines(w, h, f, sigma, m=None, strength=1.0):
+        es = [Engine(default_params(sigma, bt=3, bw=48, bh=48, ow=16, oh=16, sharpen=0.0), fmt, w, h, f, p)
+              for p in range(3)]
+        if m is not None:
+            for e in es:
+                L.set_learned(e, m, strength)
+        return es
+
+    def filt(noisy, sigma, m=None, strength=1.0):
+        f, h, w = noisy[0].shape
+        es = engines(w, h, f, sigma, m, strength)
+        outs = []
+        for p, e in enumerate(es):
+            s_ = torch.from_numpy(np.ascontiguousarray(noisy[p])).cuda()
+            d_ = torch.zeros_like(s_)
+            torch.cuda.synchronize()
+            e.run_device(0, f, [s_[k].data_ptr() for k in range(f)], w * 4, [d_[k].data_ptr() for k in range(f)],
+                         w * 4)
+            e.sync()
+            outs.append(d_.cpu().numpy())
+            e.close()
+        return outs
+
+    def score(outs, clean):
+        mse = np.mean([np.mean((o.astype(np.float64) - c) ** 2) for o, c in zip(outs, clean)])
+        return float(10 * np.log10(1.0 / mse))
+    tc, tn, _ = L.noisy_clip(384, 288, 5, seed=4000)
+    best_sigma = max([7.0, 8.0, 9.0, 10.0], key=lambda s: score(filt(tn, s), tc))
+    best_strength = max([2.5, 3.0, 3.5], key=lambda a: score(filt(tn, 2.0, mlp, a), tc))
+    plain_db, learned_db = [], []
+    for k in range(2):
+        c, n, _ = L.noisy_clip(384, 288, 5, seed=5000 + k)
+        plain_db.append(score(filt(n, best_sigma), c))
+        learned_db.append(score(filt(n, 2.0, mlp, best_strength), c))
+    # throughput at 1920x1080, resident clip (seeded content, white noise of
+    # the same level: the timing does not depend on the noise process)
+    F = args.learned_frames
+    clean = L.clean_planes(1920, 1080, F, seed=6000)
+    rng = np.random.default_rng(6000)
+    src = [torch.from_numpy((pl + rng.normal(0, 0.02, pl.shape)).astype(np.float32)).cuda() for pl in clean]
+    dst = [torch.zeros_like(s_) for s_ in src]
+    stride = 1920 * 4
+
+    def timed(es, batch):
+        items = [(e, 0, F, [src[p][k].data_ptr() for k in range(F)], stride, [dst[p][k].data_ptr() for k in range(F)],
+                  stride) for p, e in enumerate(es)]
+        torch.cuda.synchronize()
+        if batch:
+            run_device_batch(items)
+        else:
+            for it in items:
+                it[0].run_device(*it[1:])
+        for e in es:
+            e.sync()
+        t = time.perf_counter()
+        for _ in range(2):
+            if batch:
+                run_device_batch(items)
+            else:
+                for it in items:
+                    it[0].run_device(*it[1:])
+        for e in es:
+            e.sync()
+        torch.cuda.synchronize()
+        return 2 * F / (time.perf_counter() - t)
+    es = engines(1920, 1080, F, best_sigma)
+    fps_plain = timed(es, True)
+    for e in es:
+        e.close()
+    es = engines(1920, 1080, F, 2.0, mlp, best_strength)
+    fps_learned = timed(es, False)
+    for e in es:
+        e.close()
+    # MLP inference rate alone (one 1080p plane's blocks)
+    nox, noy = plane_geometry(1920, 1080, 48, 48, 16, 16)
+    nb = nox * noy
+    xb = torch.zeros((nb, mlp.n_in), dtype=torch.bfloat16, device="cuda")
+    out = torch.empty((nb, mlp.n_out), dtype=torch.float32, device="cuda")
+    work = torch.empty(mlp.work_bytes(nb), dtype=torch.uint8, device="cuda")
+    mlp.predict(xb.data_ptr(), nb, out.data_ptr(), work.data_ptr(), exp_out=True)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(20):
+        mlp.predict(xb.data_ptr(), nb, out.data_ptr(), work.data_ptr(), exp_out=True)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / 20
+    flops = 2 * nb * (mlp.n_in * mlp.nh + mlp.nh * mlp.nh + mlp.nh * mlp.n_out)
+    return {
+        "workload": "rgbps_float_bt3_bw48_ow16, coloured noise (power law x anisotropy, Perlin sigma 0.005-0.04)",
+        "train": {"blocks": data.rows, "steps": 1500, "batch": 4096, "seconds": round(t_train, 1),
+                  "whittle_loss": [round(losses[0][1], 4), round(losses[-1][1], 4)]},
+        "psnr_db_plain": [round(v, 2) for v in plain_db], "psnr_db_learned": [round(v, 2) for v in learned_db],
+        "psnr_gain_db": round(float(np.mean(learned_db) - np.mean(plain_db)), 2),
+        "best_sigma": best_sigma, "best_strength": best_strength,
+        "fps_1080p_plain": round(fps_plain, 1), "fps_1080p_learned": round(fps_learned, 1),
+        "timing_frames": F,
+        "mlp_forward": {"blocks": nb, "ms": round(ms, 4), "tflops_bf16": round(flops / (ms * 1e-3) / 1e12, 1),
+                        "frac_of_2500": round(flops / (ms * 1e-3) / 1e12 / 2500.0, 4)},
+    }
+
+
 # ------------------------------------------------------------- CPU baseline
 def cpu_worker(args):
     """Runs in a child process (MKL threading layer set before it loads)."""
@@ -300,6 +425,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-precision-form", action="store_true",
                     help="skip the DFT-as-GEMM (MFMA, FP16 split) transform measurement")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-PSD leg")
+    ap.add_argument("--learned-frames", type=int, default=24)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run one device copy of MIB MiB first (a reference kernel for traffic counters)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -330,6 +457,8 @@ def main():
     }
     if not args.no_precision_form:
         line["precision_form"] = precision_form(args)
+    if not args.no_learned:
+        line["learned_psd"] = learned_form(args)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args)
         cb = line["cpu_baseline"]["value"]
