@@ -64,6 +64,23 @@ def frame_plane_bytes(pw, ph, bt, bps=1, bw=48, bh=48, ow=16, oh=16):
     return 2 * pw * ph * bps + r * spec
 
 
+TRAFFIC_PROFILE = "profiles/k_run_yuv420p8_1080p_bt3_hbm_traffic.md"
+
+
+def measured_traffic():
+    """HBM bytes per frame of k_run + k_seam_run from the committed rocprofv3
+    counter summary (a text file: `key: value` lines)."""
+    path = os.path.join(ROOT, TRAFFIC_PROFILE)
+    if not os.path.exists(path):
+        return None
+    out = {}
+    for line in open(path):
+        if line.startswith(("traffic_bytes_per_frame:", "algorithmic_bytes_per_frame:")):
+            k, v = line.split(":")
+            out[k.strip()] = float(v)
+    return out if len(out) == 2 else None
+
+
 def psnr(a, b, peak=255.0):
     import numpy as np
     d = a.astype(np.float64) - b.astype(np.float64)
@@ -158,11 +175,21 @@ def gpu_run(args):
         "noisy_luma": round(float(np.mean([psnr(planes[0][n], clean[0][n]) for n in idx])), 3),
         "denoised_luma": round(float(np.mean([psnr(out_y[n], clean[0][n]) for n in idx])), 3),
     }
+    # measured HBM traffic of the same launches (rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE with a copy calibration; summary and commands in profiles/)
+    traffic = traffic_ratio = None
+    if args.bt == 3:
+        t = measured_traffic()
+        if t:
+            traffic = int(t["traffic_bytes_per_frame"] * 16)  # one full 16-frame launch
+            traffic_ratio = round(t["traffic_bytes_per_frame"] / t["algorithmic_bytes_per_frame"], 3)
     return {
         "fps": fps, "elapsed": elapsed, "ms_per_step": elapsed * 1e3 / args.steps, "gen_s": gen_s,
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "traffic_over_algorithmic": traffic_ratio,
+            "traffic_source": TRAFFIC_PROFILE + " (FETCH_SIZE x2.00, WRITE_SIZE x1.00 by a 1 GiB copy)",
             "kernel": f"k_run<48,48,MODE_W3D,bt={args.bt},sharpen> + k_seam_run (3 planes per launch)",
             "us_per_frame_plane": round(avg_us, 2), "frame_planes": frames_timed,
             "bytes_model": "per frame-plane: 2*W*H*Bps + bt*nb*bh*(bw/2+1)*8",
