@@ -182,6 +182,23 @@ int nf3d_engine_spectra(nf3d_engine* e, const void* src, int src_stride, void* s
  * an engine uses it.  Frames then go through the per-frame path. */
 int nf3d_engine_set_learned(nf3d_engine* e, nf3d_mlp* m, float strength, char* err, size_t errlen);
 
+/* ------------------------------------------------------------ launch tuning
+ * No reference counterpart (the reference's `opt` picks an ISA; these pick
+ * launch shapes, and no choice changes a result).  The frame-loop path runs
+ * frames_per_launch frames per kernel launch (1..16) and, for Wiener 2D and
+ * sharpen-only (no temporal recursion), frames_per_workgroup frames per
+ * workgroup (Wiener 3D and Kalman keep a launch's frames in one workgroup).
+ * A process-wide table, one line per "bw bh bits bt frames_per_launch
+ * frames_per_workgroup" ('#' comments; bits 32 = float), is read from the file
+ * named by NF3D_TUNE_TABLE or by nf3d_tune_load (which replaces it and returns
+ * its line count); engines created afterwards take their entry.
+ * neo_fft3d_amd/autotune.py measures the choices and writes such a table. */
+int nf3d_tune_load(const char* path, char* err, size_t errlen);
+int nf3d_tune_lookup(int bw, int bh, int bits, int bt, int* out2);  /* 1 if found */
+int nf3d_engine_set_tuning(nf3d_engine* e, int frames_per_launch, int frames_per_workgroup, char* err,
+                           size_t errlen);
+int nf3d_engine_get_tuning(nf3d_engine* e, int* out2);
+
 /* Replaces FFT3DEngine::CacheRefresh (src/fft3d_engine.h:736-740): 1 if the
  * spectrum of frame n is held in the HBM spectra ring (and marks it recent). */
 int nf3d_engine_cache_refresh(nf3d_engine* e, int n);

@@ -58,6 +58,61 @@ void copy_err(char* err, size_t len, const std::string& m) {
   }
 }
 
+// Launch tuning of the frame-loop path (results do not depend on it): frames
+// per k_run launch and, for the modes without a temporal recursion (Wiener
+// 2D, sharpen), frames per workgroup.  A process-wide table keyed by block
+// shape, bit depth and bt, loaded by nf3d_tune_load or from the file named
+// by NF3D_TUNE_TABLE; engines take their entry when created.
+struct TuneEntry {
+  int bw, bh, bits, bt, frames, fpw;
+};
+std::mutex g_tune_mu;
+std::vector<TuneEntry> g_tune;
+bool g_tune_env_done = false;
+
+int parse_tune_file(const char* path, std::vector<TuneEntry>& out, std::string& msg) {
+  FILE* f = fopen(path, "r");
+  if (!f) {
+    msg = std::string("cannot open tuning table ") + path;
+    return -1;
+  }
+  char line[256];
+  int ln = 0;
+  out.clear();
+  while (fgets(line, sizeof line, f)) {
+    ln++;
+    const char* q = line;
+    while (*q == ' ' || *q == '\t') q++;
+    if (*q == '#' || *q == '\n' || *q == 0) continue;
+    TuneEntry t{};
+    if (sscanf(q, "%d %d %d %d %d %d", &t.bw, &t.bh, &t.bits, &t.bt, &t.frames, &t.fpw) != 6 || t.frames < 1 ||
+        t.frames > nf3d::RUN_MAX_FRAMES || t.fpw < 1 || t.fpw > nf3d::RUN_MAX_FRAMES) {
+      fclose(f);
+      msg = std::string("tuning table ") + path + ": bad line " + std::to_string(ln);
+      return -1;
+    }
+    out.push_back(t);
+  }
+  fclose(f);
+  return (int)out.size();
+}
+
+bool tune_lookup(int bw, int bh, int bits, int bt, TuneEntry* hit) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  if (!g_tune_env_done) {
+    g_tune_env_done = true;
+    const char* path = getenv("NF3D_TUNE_TABLE");
+    std::string msg;
+    if (path && *path && parse_tune_file(path, g_tune, msg) < 0) fprintf(stderr, "nf3d: %s (ignored)\n", msg.c_str());
+  }
+  for (const TuneEntry& t : g_tune)
+    if (t.bw == bw && t.bh == bh && t.bits == bits && t.bt == bt) {
+      *hit = t;
+      return true;
+    }
+  return false;
+}
+
 template <typename T>
 T* dev_alloc(size_t count) {
   void* p = nullptr;
@@ -113,7 +168,9 @@ struct nf3d_engine {
   // (consecutive launches alternate, so a launch does not wait on the seam
   // pass of the one before for its buffer)
   float* d_seam_run = nullptr;
-  static constexpr int kRunFrames = nf3d::RUN_MAX_FRAMES;  // frames per k_run launch
+  static constexpr int kRunFrames = nf3d::RUN_MAX_FRAMES;  // largest k_run launch (frames)
+  int run_frames = kRunFrames;  // frames per k_run launch (tuning)
+  int run_fpw = 1;              // frames per workgroup, Wiener 2D / sharpen (tuning)
   // learned PSD: model, strength, the frame's spectra, features, per-block
   // pattern [nb][bh][outwidth] and the model's inference workspace
   nf3d_mlp* learned = nullptr;
@@ -390,6 +447,11 @@ struct nf3d_engine {
     d_seam = dev_alloc<float>((size_t)4 * Hc * Wc);
     int bpw = nf3d::tile_blocks_per_wg(bw, bh);
     ntx = (nox + bpw - 1) / bpw;
+    TuneEntry tune;
+    if (tune_lookup(bw, bh, fmt.is_float ? 32 : fmt.bits_per_sample, ep.bt, &tune)) {
+      run_frames = tune.frames;
+      run_fpw = tune.fpw;
+    }
     if (ep.bt >= 2) {
       ring_slots = ep.bt + 2;
       d_ring = dev_alloc<float2>(spec_elems * ring_slots);
@@ -844,7 +906,7 @@ struct nf3d_engine {
     const size_t seam_frame = (size_t)4 * Hc * Wc;
     if (!d_seam_run) d_seam_run = dev_alloc<float>(2 * kRunFrames * seam_frame);
     const int total = P.b - P.a + 1;
-    const int nl = (total + kRunFrames - 1) / kRunFrames;
+    const int nl = (total + run_frames - 1) / run_frames;
     for (int j = 0, m0 = P.a; j < nl; j++) {
       const int nf = total / nl + (j >= nl - total % nl ? 1 : 0);
       RunLaunch R{};
@@ -861,7 +923,7 @@ struct nf3d_engine {
       R.spec_elems = (long long)spec_elems;
       R.m0 = m0;
       R.nframes = nf;
-      R.fpw = (P.mode == nf3d::MODE_W3D || P.mode == nf3d::MODE_KALMAN) ? nf : 1;
+      R.fpw = (P.mode == nf3d::MODE_W3D || P.mode == nf3d::MODE_KALMAN) ? nf : std::min(run_fpw, nf);
       R.ntiles = noy * ntx;
       P.launches.push_back(R);
       m0 += nf;
@@ -1235,6 +1297,52 @@ int nf3d_engine_set_learned(nf3d_engine* e, nf3d_mlp* m, float strength, char* e
     copy_err(err, errlen, f.msg);
     return -1;
   }
+  return 0;
+}
+
+int nf3d_tune_load(const char* path, char* err, size_t errlen) {
+  std::vector<TuneEntry> t;
+  std::string msg;
+  if (!path) {
+    copy_err(err, errlen, "null path");
+    return -1;
+  }
+  int n = parse_tune_file(path, t, msg);
+  if (n < 0) {
+    copy_err(err, errlen, msg);
+    return -1;
+  }
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  g_tune = std::move(t);
+  g_tune_env_done = true;
+  return n;
+}
+
+int nf3d_tune_lookup(int bw, int bh, int bits, int bt, int* out2) {
+  TuneEntry t;
+  if (!tune_lookup(bw, bh, bits, bt, &t)) return 0;
+  out2[0] = t.frames;
+  out2[1] = t.fpw;
+  return 1;
+}
+
+int nf3d_engine_set_tuning(nf3d_engine* e, int frames_per_launch, int frames_per_workgroup, char* err,
+                           size_t errlen) {
+  if (frames_per_launch < 1 || frames_per_launch > nf3d::RUN_MAX_FRAMES || frames_per_workgroup < 1 ||
+      frames_per_workgroup > nf3d::RUN_MAX_FRAMES) {
+    copy_err(err, errlen, "frames per launch and per workgroup must be 1.." + std::to_string(nf3d::RUN_MAX_FRAMES));
+    return -1;
+  }
+  std::lock_guard<std::mutex> lock(e->mu);
+  e->run_frames = frames_per_launch;
+  e->run_fpw = frames_per_workgroup;
+  return 0;
+}
+
+int nf3d_engine_get_tuning(nf3d_engine* e, int* out2) {
+  std::lock_guard<std::mutex> lock(e->mu);
+  out2[0] = e->run_frames;
+  out2[1] = e->run_fpw;
   return 0;
 }
 
