@@ -179,7 +179,11 @@ int nf3d_engine_spectra(nf3d_engine* e, const void* src, int src_stride, void* s
 /* Use the model's per-block PSD (times `strength`) as the noise pattern of
  * every frame (null model: back to the reference's sigma / pattern).  The
  * model must be n_in = n_out = bh * (bw/2+1) and must not be trained while
- * an engine uses it.  Frames then go through the per-frame path. */
+ * an engine uses it.  The per-frame path predicts each frame's PSD from that
+ * frame; the frame-loop path (nf3d_engine_run_device, the batch calls) predicts
+ * it once per launch, from the launch's first frame, for all the launch's
+ * frames (frames per launch: nf3d_engine_set_tuning; 1 gives the per-frame
+ * semantics). */
 int nf3d_engine_set_learned(nf3d_engine* e, nf3d_mlp* m, float strength, char* err, size_t errlen);
 
 /* ------------------------------------------------------------ launch tuning

@@ -736,23 +736,32 @@ struct nf3d_engine {
     launch(nf3d::MODE_SHARPEN, 0, J, c.s, true);
   }
 
-  // Learned PSD of frame n's blocks: the frame's own analysis, features,
-  // exp(model log-PSD) * mult into d_lpat; J reads it as a per-block pattern.
-  void learned_psd(int n, const Call& c, float mult, PlaneJob& J) {
-    Src s0 = source(c, n);
+  // Learned PSD of one plane's blocks: the plane's own analysis, features,
+  // exp(model log-PSD) * mult * strength into d_lpat.
+  void learned_psd_plane(const uint8_t* src, long long pitch, float mult, hipStream_t s) {
     PlaneJob F = base_job();
-    F.src = s0.p;
-    F.src_pitch = s0.pitch;
+    F.src = src;
+    F.src_pitch = pitch;
     F.spec_out = d_lspec;
-    launch(nf3d::MODE_FWD, 0, F, c.s);
-    if (nf3d_mlp_features(learned, d_lspec, nb, d_lx, c.s)) fail("learned PSD: feature kernel");
+    launch(nf3d::MODE_FWD, 0, F, s);
+    if (nf3d_mlp_features(learned, d_lspec, nb, d_lx, s)) fail("learned PSD: feature kernel");
     char msg[256] = {0};
-    if (nf3d_mlp_predict(learned, d_lx, nb, d_lpat, 1, mult * learned_strength, d_lwork, c.s, msg, sizeof msg))
+    if (nf3d_mlp_predict(learned, d_lx, nb, d_lpat, 1, mult * learned_strength, d_lwork, s, msg, sizeof msg))
       fail(std::string("learned PSD: ") + msg);
+  }
+
+  void use_learned(PlaneJob& J) const {
     J.use_pattern = 1;
     J.pattern_per_block = 1;
     J.pattern2d = d_lpat;
     J.pattern3d = d_lpat;
+  }
+
+  // Per-frame path: the PSD of frame n itself.
+  void learned_psd(int n, const Call& c, float mult, PlaneJob& J) {
+    Src s0 = source(c, n);
+    learned_psd_plane(s0.p, s0.pitch, mult, c.s);
+    use_learned(J);
   }
 
   void run(int n, const Call& c, uint8_t* dst, long long dst_pitch) {
@@ -899,7 +908,7 @@ struct nf3d_engine {
                       (long long)(tc + Hc) * dst_pitch < (1LL << 31) && 4LL * Hc * Wc < (1LL << 31) &&
                       dst_pitch % pix == 0;
     const bool pshow = ep.pfactor != 0 && ep.pshow;
-    P.run = P.a <= P.b && fits && !pshow && !learned && !(P.mode == nf3d::MODE_SHARPEN && P.shp == nf3d::SHP_NONE);
+    P.run = P.a <= P.b && fits && !pshow && !(P.mode == nf3d::MODE_SHARPEN && P.shp == nf3d::SHP_NONE);
     if (!P.run) return P;
     // launches of equal length (within one frame), two halves of seam
     // buffers in turn
@@ -913,6 +922,9 @@ struct nf3d_engine {
       R.J = base_job();
       R.J.sigma2 = (float)(P.mode == nf3d::MODE_W3D ? ep.bt : 1) * ep.sigma * ep.sigma / norm;
       R.J.seam = d_seam_run + (size_t)(j & 1) * kRunFrames * seam_frame;
+      // learned PSD: one per launch, from the launch's first frame (the
+      // engine writes it to d_lpat just before the launch)
+      if (learned) use_learned(R.J);
       R.J.src_pitch = src_pitch;
       R.J.dst_pitch = dst_pitch;
       for (int i = 0; i < nf; i++) {
@@ -944,7 +956,7 @@ struct nf3d_engine {
       estimate_pattern(Call{&r, 1, true, s});
     }
     if (P.mode == nf3d::MODE_W3D) {
-      if (ep.pfactor != 0) ensure_pattern3d(ep.bt, s);
+      if (ep.pfactor != 0 && !learned) ensure_pattern3d(ep.bt, s);
       // the window's older frames of the first step, into slot (frame % bt)
       for (int k = P.a + P.from; k < P.a + P.to; k++) {
         PlaneJob F = base_job();
@@ -1104,6 +1116,9 @@ void run_batch(const nf3d_run_item* items, int n, hipStream_t s, bool timed) {
           nf3d_engine* e = items[k].engine;
           RunLaunch& R = B.job[B.njobs++];
           R = plans[k].launches[l];
+          if (e->learned)
+            e->learned_psd_plane(static_cast<const uint8_t*>(plans[k].src[R.m0]), plans[k].src_pitch,
+                                 plans[k].mode == nf3d::MODE_W3D ? (float)e->ep.bt : 1.0f, s);
           R.wg_begin = B.total_wg;
           B.total_wg += R.ntiles * ((R.nframes + R.fpw - 1) / R.fpw);
           B.max_hc = std::max(B.max_hc, e->Hc);

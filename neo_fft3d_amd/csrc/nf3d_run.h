@@ -426,7 +426,10 @@ __global__ void __launch_bounds__(NT, 2) k_run(RunBatch B_arg) {
     const int rb = row_on ? rbr : nblk - 1;
     const size_t tile_spec0 = (size_t)(by * J.nox + bx0) * (BH * COLS);
     const size_t colo = tile_spec0 + (size_t)cb * (BH * COLS) + cc;  // element (0, cc) of this lane's column
-    const float* pat = J.use_pattern ? (MODE == MODE_W3D ? J.pattern3d : J.pattern2d) : nullptr;
+    // the pattern: one for the plane, or this block's (learned PSD)
+    const float* pat = J.use_pattern ? (MODE == MODE_W3D ? J.pattern3d : J.pattern2d) +
+                                           (J.pattern_per_block ? colo - cc : 0)
+                                     : nullptr;
     const float grid0 = J.grid[0].x;
     const SpecParams SP(J);
 

@@ -298,17 +298,43 @@ def test_learned_psd_gains_psnr(torch):
     assert min(gains) > 0.0
 
 
-@pytest.mark.parametrize("family,bits,is_float,bt", [("rgb", 32, True, 3), ("gray", 8, False, 1),
-                                                     ("gray", 8, False, 0), ("rgb", 32, True, 5)])
-def test_learned_path_matches_oracle(torch, family, bits, is_float, bt):
+def psd_source_frame(n, bt, F, per_launch):
+    """The frame whose PSD the engine uses for frame n of an in-order device
+    range 0..F-1: frames of the frame-loop range take their launch's first
+    frame (launches split as the engine splits them), the others their own."""
+    if bt >= 2:
+        lo, hi = bt // 2, F - 1 - (bt - 1) // 2
+    elif bt == 0:
+        lo, hi = 1, F - 1
+    else:
+        lo, hi = 0, F - 1
+    if n < lo or n > hi:
+        return n
+    total = hi - lo + 1
+    nl = (total + per_launch - 1) // per_launch
+    m0 = lo
+    for j in range(nl):
+        nf = total // nl + (1 if j >= nl - total % nl else 0)
+        if n < m0 + nf:
+            return m0
+        m0 += nf
+    raise AssertionError("frame outside the launches")
+
+
+@pytest.mark.parametrize("family,bits,is_float,bt,per_launch", [
+    ("rgb", 32, True, 3, 1), ("gray", 8, False, 1, 1), ("gray", 8, False, 0, 1), ("rgb", 32, True, 5, 1),
+    ("rgb", 32, True, 3, 16), ("gray", 8, False, 1, 4), ("gray", 8, False, 0, 3)])
+def test_learned_path_matches_oracle(torch, family, bits, is_float, bt, per_launch):
     """The engine's learned-PSD filtering against the FP64 oracle given the
-    same per-block PSDs (the model's exp(log-PSD) of each frame's analysis):
-    per-block pattern indexing, the exp epilogue and the btcur scaling."""
+    same per-block PSDs (the model's exp(log-PSD) of the frame the engine
+    predicts from): per-block pattern indexing in both kernels, the exp
+    epilogue, the btcur scaling and the one-PSD-per-launch rule."""
     import ctypes
     from harness import Clip, compare, oracle_runner
     from neo_fft3d_amd import Engine, default_params, make_format
     from neo_fft3d_amd.learned import Mlp, engine_spectra, noisy_clip
-    sigma, F, W, H = 6.0, 6, 160, 112
+    from neo_fft3d_amd.autotune import set_tuning
+    sigma, F, W, H = 6.0, 9, 160, 112
     _, noisy, _ = noisy_clip(W, H, F, seed=91)
     nplanes = 3 if family == "rgb" else 1
     if is_float:
@@ -361,6 +387,7 @@ def test_learned_path_matches_oracle(torch, family, bits, is_float, bt):
     gpu = []
     for p in range(nplanes):
         set_learned(engines[p], m, 1.0)
+        set_tuning(engines[p], per_launch, 1)
         dst = torch.zeros_like(srcs[p])
         torch.cuda.synchronize()  # the engine's stream does not wait for torch's
         stride = srcs[p][0].stride(0) * srcs[p].element_size()
@@ -373,12 +400,14 @@ def test_learned_path_matches_oracle(torch, family, bits, is_float, bt):
     orc.lib.nf3d_oracle_set_block_psd.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
     h = orc.open(args, clip)
     for n in range(F):
+        src_n = psd_source_frame(n, bt, F, per_launch)
         for p in range(nplanes):
-            assert orc.lib.nf3d_oracle_set_block_psd(h.h, p, psds[(p, n)].ctypes.data) == 0
+            assert orc.lib.nf3d_oracle_set_block_psd(h.h, p, psds[(p, src_n)].ctypes.data) == 0
         want = h.get(n)
         got = [gpu[p][n] for p in range(nplanes)]
         for p, (mx, fe, rms) in enumerate(compare(got, want, is_float)):
-            print(f"learned {family}{bits} bt={bt} n={n} p={p}: max {mx:.3g} exact {fe:.5f} rms {rms:.3g} "
+            print(f"learned {family}{bits} bt={bt} per_launch={per_launch} n={n} (PSD of frame {src_n}) p={p}: "
+                  f"max {mx:.3g} exact {fe:.5f} rms {rms:.3g} "
                   f"(log-PSD spread {spread:.2f})", flush=True)
             if is_float:
                 assert mx <= 1e-4
