@@ -242,10 +242,56 @@ def precision_form(args):
     # FP16 MACs per block and direction: 3 split products x (32*64*32 + 2*32*64*32)
     flops = 2 * 2 * 3 * (32 * 64 * 32 + 2 * 32 * 64 * 32) * nb
     tflops = flops / (ms * 1e-3) / 1e12
-    return {"blocks": nb, "ms_forward_plus_inverse": round(ms, 4), "blocks_per_s": round(nb / (ms * 1e-3)),
+    filt = precision_filter(args)
+    return {"filter": filt, "blocks": nb, "ms_forward_plus_inverse": round(ms, 4), "blocks_per_s": round(nb / (ms * 1e-3)),
             "mfma_tflops_fp16": round(tflops, 1), "mfma_peak_tflops_fp16": 2500.0,
             "frac": round(tflops / 2500.0, 4), "forward_max_err_rel_fp64": fwd_err,
             "round_trip_max_err_10bit": rt_err}
+
+
+def precision_filter(args):
+    """The filter itself in the precision form (dft=1: its 2D transforms as
+    matrix-core DFT products) on YUV420P10 3840x2160, bt=3, 32/16: frames/s
+    of both forms on a resident clip and how often their outputs agree.
+    (The error of each form against FP64 is measured by the tests.)"""
+    import numpy as np
+    import torch
+
+    from neo_fft3d_amd import Engine, default_params, lib, make_format, run_device_batch
+    from neo_fft3d_amd.synth import SEED_BASE, rects_clip
+    L = lib()
+    L.nf3d_engine_set_dft.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
+    F, w, h = args.precision_frames, 3840, 2160
+    planes = rects_clip(w, h, F, SEED_BASE + 3, bits=10, noise_luma=3.0, noise_chroma=3.0)
+    src = [torch.from_numpy(p).cuda() for p in planes]
+    fmt = make_format("yuv", 10)
+    out, fps = {}, {}
+    for dft in (0, 1):
+        es = [Engine(default_params(3.0, bt=3, bw=32, bh=32, ow=16, oh=16), fmt, w, h, F, p) for p in range(3)]
+        if dft:
+            for e in es:
+                assert L.nf3d_engine_set_dft(e.h, 1, None, 0) == 0
+        dst = [torch.zeros_like(x) for x in src]
+        items = [(e, 0, F, [src[p][k].data_ptr() for k in range(F)], src[p][0].stride(0) * 2,
+                  [dst[p][k].data_ptr() for k in range(F)], src[p][0].stride(0) * 2) for p, e in enumerate(es)]
+        torch.cuda.synchronize()
+        run_device_batch(items)
+        for e in es:
+            e.sync()
+        out[dft] = [d.cpu().numpy() for d in dst]
+        t = time.perf_counter()
+        for _ in range(2):
+            run_device_batch(items)
+        for e in es:
+            e.sync()
+        fps[dft] = 2 * F / (time.perf_counter() - t)
+        for e in es:
+            e.close()
+    d = [np.abs(a.astype(np.int32) - b.astype(np.int32)) for a, b in zip(out[0], out[1])]
+    return {"workload": "yuv420p10_3840x2160_bt3_bw32_ow16", "frames": F,
+            "fps_fft_frame_loop": round(fps[0], 1), "fps_mfma_dft_per_frame": round(fps[1], 1),
+            "forms_max_diff_lsb": int(max(x.max() for x in d)),
+            "forms_equal_frac": round(float(np.mean([np.mean(x == 0) for x in d])), 6)}
 
 
 def learned_form(args):
@@ -456,6 +502,7 @@ def main():
                     help="skip the DFT-as-GEMM (MFMA, FP16 split) transform measurement")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-PSD leg")
     ap.add_argument("--learned-frames", type=int, default=48)
+    ap.add_argument("--precision-frames", type=int, default=8)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run one device copy of MIB MiB first (a reference kernel for traffic counters)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)

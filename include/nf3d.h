@@ -186,6 +186,13 @@ int nf3d_engine_spectra(nf3d_engine* e, const void* src, int src_stride, void* s
  * semantics). */
 int nf3d_engine_set_learned(nf3d_engine* e, nf3d_mlp* m, float strength, char* err, size_t errlen);
 
+/* Precision form of the engine's 2D transforms (no reference counterpart; the
+ * reference uses FFTW): mode 0 = FP32 FFT in registers (default), 1 = dense
+ * DFT-matrix products on the matrix cores with FP16 hi/lo split operands and
+ * FP32 accumulation (32x32 blocks; frames then go through the per-frame
+ * kernel).  Errors: other block shapes, other modes. */
+int nf3d_engine_set_dft(nf3d_engine* e, int mode, char* err, size_t errlen);
+
 /* ------------------------------------------------------------ launch tuning
  * No reference counterpart (the reference's `opt` picks an ISA; these pick
  * launch shapes, and no choice changes a result).  The frame-loop path runs

@@ -32,6 +32,7 @@
 
 #include "../../include/nf3d.h"
 #include "nf3d_internal.h"
+#include "nf3d_mfma.h"
 
 using nf3d::PatternJob;
 using nf3d::PlaneJob;
@@ -171,6 +172,11 @@ struct nf3d_engine {
   static constexpr int kRunFrames = nf3d::RUN_MAX_FRAMES;  // largest k_run launch (frames)
   int run_frames = kRunFrames;  // frames per k_run launch (tuning)
   int run_fpw = 1;              // frames per workgroup, Wiener 2D / sharpen (tuning)
+  // 2D transforms: 0 = FFT in registers, 1 = DFT-matrix products on the matrix
+  // cores (32x32 blocks; every frame through the per-frame kernel)
+  int dft_mode = 0;
+  const void* dft_tab = nullptr;
+  float dft_sx = 1.0f, dft_sz = 1.0f;
   // learned PSD: model, strength, the frame's spectra, features, per-block
   // pattern [nb][bh][outwidth] and the model's inference workspace
   nf3d_mlp* learned = nullptr;
@@ -520,6 +526,9 @@ struct nf3d_engine {
     J.kal_last = d_kal_last;
     J.kal_covar = d_kal_covar;
     J.kal_cproc = d_kal_cproc;
+    J.dft_tab = dft_mode ? dft_tab : nullptr;
+    J.dft_sx = dft_sx;
+    J.dft_sz = dft_sz;
     return J;
   }
 
@@ -585,7 +594,7 @@ struct nf3d_engine {
     tl.total_seam_rows = Hc;
     tl.max_wc = Wc;
     auto ev = timer(timed, 1, s);
-    check(nf3d::launch_tile(mode, bt, onchip, ep.bw, ep.bh, tl, s), "tile kernel");
+    check(nf3d::launch_tile(mode, bt, onchip, dft_mode, ep.bw, ep.bh, tl, s), "tile kernel");
     if (ev.second) check(hipEventRecord(ev.second, s), "hipEventRecord");
   }
 
@@ -908,7 +917,7 @@ struct nf3d_engine {
                       (long long)(tc + Hc) * dst_pitch < (1LL << 31) && 4LL * Hc * Wc < (1LL << 31) &&
                       dst_pitch % pix == 0;
     const bool pshow = ep.pfactor != 0 && ep.pshow;
-    P.run = P.a <= P.b && fits && !pshow && !(P.mode == nf3d::MODE_SHARPEN && P.shp == nf3d::SHP_NONE);
+    P.run = P.a <= P.b && fits && !pshow && !dft_mode && !(P.mode == nf3d::MODE_SHARPEN && P.shp == nf3d::SHP_NONE);
     if (!P.run) return P;
     // launches of equal length (within one frame), two halves of seam
     // buffers in turn
@@ -1308,6 +1317,32 @@ int nf3d_engine_set_learned(nf3d_engine* e, nf3d_mlp* m, float strength, char* e
     e->d_lwork = dev_alloc<uint8_t>((size_t)nf3d_mlp_work_bytes(m, e->nb));
     e->learned = m;
     e->learned_strength = strength;
+  } catch (const Failure& f) {
+    copy_err(err, errlen, f.msg);
+    return -1;
+  }
+  return 0;
+}
+
+int nf3d_engine_set_dft(nf3d_engine* e, int mode, char* err, size_t errlen) {
+  std::lock_guard<std::mutex> lock(e->mu);
+  try {
+    if (mode != 0 && mode != 1) fail("dft mode must be 0 (FFT) or 1 (matrix-core DFT)");
+    if (mode == 1 && !nf3d::mfma_dft_supported(e->ep.bw, e->ep.bh))
+      fail("the matrix-core DFT supports 32x32 blocks");
+    check(hipSetDevice(e->device), "hipSetDevice");
+    if (mode == 1 && !e->dft_tab) {
+      hipError_t he;
+      e->dft_tab = nf3d::mfma_dft_table(&he);
+      check(he, "matrix-core DFT tables");
+    }
+    // power-of-two operand scales keeping every FP16 operand below 2^14:
+    // windowed samples up to the format's range (float: 4), spectra up to
+    // bw*bh times that with headroom 4 for the gains
+    const double maxabs = e->fmt.is_float ? 4.0 : (double)((1 << e->fmt.bits_per_sample) - 1);
+    e->dft_sx = std::ldexp(1.0f, (int)std::floor(std::log2(16384.0 / maxabs)));
+    e->dft_sz = std::ldexp(1.0f, (int)std::floor(std::log2(16384.0 / (1024.0 * maxabs * 4.0))));
+    e->dft_mode = mode;
   } catch (const Failure& f) {
     copy_err(err, errlen, f.msg);
     return -1;

@@ -242,6 +242,15 @@ int nf3d_filter_create(const char* args, const nf3d_format* fmt, int width, int 
         return -1;
       }
       f->engine_count++;
+      // "dft=1": the matrix-core DFT form (an extension; not part of the
+      // reference's parameter list or signatures)
+      int dft = 0;
+      in.read("dft", dft);
+      if (dft && nf3d_engine_set_dft(f->eng[i], dft, &e[0], e.size()) != 0) {
+        put_err(err, errlen, e.c_str());
+        delete f;
+        return -1;
+      }
     } else if (f->process[i] == 2) {
       f->copy_count++;
     }

@@ -58,6 +58,10 @@ struct PlaneJob {
   float sharpen, smin_n, smax_n, dehalo, ht2n, kratio2;
   int use_pattern, use_degrid;
   int pattern_per_block;  // pattern2d / pattern3d hold one [bh][outwidth] pattern per block (learned PSD)
+  // matrix-core DFT (32x32 blocks, per-frame kernel): constant tiles and the
+  // power-of-two operand scales of the forward and inverse transforms
+  const void* dft_tab;
+  float dft_sx, dft_sz;
   // spectra, layout [block][bh][outwidth] float2 (compact)
   float2* spec_out;          // where this launch's forward spectra are stored (or null)
   const float2* spec_in[5];  // Wiener-3D window, oldest first (on-chip newest: unused)
@@ -126,7 +130,9 @@ struct PatternJob {
 // Launchers; return hipSuccess or the launch error.  `bw`/`bh` select the
 // compiled block shape; unsupported shapes return hipErrorInvalidValue.
 // launch_tile runs k_tile and, for the filtering modes, k_seam.
-hipError_t launch_tile(int mode, int bt, int onchip, int bw, int bh, const TileLaunch& l, hipStream_t s);
+hipError_t launch_tile(int mode, int bt, int onchip, int mf, int bw, int bh, const TileLaunch& l, hipStream_t s);
+// Block shapes with a matrix-core DFT variant of k_tile (mf = 1).
+bool mfma_dft_supported(int bw, int bh);
 hipError_t launch_pattern(const PatternJob& job, hipStream_t s);
 // DFT-as-GEMM precision form (nf3d_mfma_dft.hip): n x n blocks, n = 32.
 hipError_t launch_dft_mfma(const void* in, void* out, int nb, int n, int inverse, float scale, hipStream_t s);

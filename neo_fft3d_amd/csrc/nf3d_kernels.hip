@@ -64,7 +64,7 @@ __global__ void __launch_bounds__(256) k_pattern_set(PatternJob J) {
 #define NF3D_SHAPES(X) X(8, 8) X(16, 16) X(24, 24) X(32, 32) X(48, 48) X(64, 64) X(32, 16) X(16, 32)
 
 #define X(W, H)                                                                                          \
-  hipError_t launch_tile_##W##x##H(int mode, int bt, int onchip, const TileLaunch& l, hipStream_t s); \
+  hipError_t launch_tile_##W##x##H(int mode, int bt, int onchip, int mf, const TileLaunch& l, hipStream_t s); \
   hipError_t launch_run_##W##x##H(int mode, int bt, int shp, const RunBatch& l, hipStream_t s); \
   hipError_t launch_seam_run_##W##x##H(const RunBatch& l, hipStream_t s);
 NF3D_SHAPES(X)
@@ -83,8 +83,11 @@ int tile_blocks_per_wg(int bw, int bh) {
   return 128 / (cols > pairs ? cols : pairs);
 }
 
-hipError_t launch_tile(int mode, int bt, int onchip, int bw, int bh, const TileLaunch& l, hipStream_t s) {
-#define X(W, H) if (bw == W && bh == H) return launch_tile_##W##x##H(mode, bt, onchip, l, s);
+bool mfma_dft_supported(int bw, int bh) { return bw == 32 && bh == 32; }
+
+hipError_t launch_tile(int mode, int bt, int onchip, int mf, int bw, int bh, const TileLaunch& l, hipStream_t s) {
+  if (mf && !mfma_dft_supported(bw, bh)) return hipErrorInvalidValue;
+#define X(W, H) if (bw == W && bh == H) return launch_tile_##W##x##H(mode, bt, onchip, mf, l, s);
   NF3D_SHAPES(X)
 #undef X
   return hipErrorInvalidValue;

@@ -7,10 +7,13 @@
 // pre-activations and Adam moments.
 //
 // Kernels:
-//   k_gemm<TA, TB, EPI>   C(m, n) = sum_k A(m, k) B(k, n) over a 64x64 tile
-//                         per workgroup (4 waves, 32x32 each = 2x2 MFMA tiles),
-//                         K in steps of 64 staged through LDS, next step's
-//                         operands loaded into registers during the MFMAs;
+//   k_gemm<TA, TB, EPI, TM, TN>
+//                         C(m, n) = sum_k A(m, k) B(k, n) over a TM x TN tile
+//                         per workgroup (4 waves of TM/2 x TN/2, 16x16 MFMA
+//                         tiles; 64x64, or 32x32 when the 64x64 grid would
+//                         leave CUs idle), K in steps of 64 staged through
+//                         LDS, next step's operands loaded into registers
+//                         during the MFMAs;
 //                         epilogues: bias + GELU, bias, bias + exp (PSD),
 //                         GELU backward, plain FP32 (weight gradients)
 //   k_features            log-power features of compact block spectra
@@ -33,7 +36,7 @@ namespace nf3d_mlp_k {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int TM = 64, TN = 64, TK = 64, NT = 256;
+constexpr int TK = 64, NT = 256;
 constexpr int LDK = TK + 8;  // LDS row: 64 k-values + 8 pad (144 bytes, 16-byte aligned)
 
 enum Epi : int {
@@ -64,23 +67,26 @@ __device__ __forceinline__ float gelu_grad(float z) {
   return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
 }
 
-// One operand tile (64 rows of the outer index x 64 k) as 512 chunks of 8
-// contiguous elements; each lane loads two.  KC: k is the contiguous index
-// in memory (row-major A, transposed B), else the outer index is.
-template <bool KC>
+// One operand tile (R rows of the outer index x 64 k) as R*8 chunks of 8
+// contiguous elements; each lane loads R*8/256 of them.  KC: k is the
+// contiguous index in memory (row-major A, transposed B), else the outer
+// index is.
+template <bool KC, int R>
 struct TileLoad {
-  uint4 v[2];
+  static constexpr int NCH = R * 8 / NT;
+  static constexpr int OC = R / 8;  // chunks per k-row when the outer index is contiguous
+  uint4 v[NCH];
   __device__ void load(const __bf16* P, long long ld, int o0, int olim, int k0, int klim, int t) {
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
+    for (int h = 0; h < NCH; h++) {
       const int c = t + h * NT;
       int o, k;
       if constexpr (KC) {
         o = c >> 3;
         k = (c & 7) * 8;
       } else {
-        k = c >> 3;
-        o = (c & 7) * 8;
+        k = c / OC;
+        o = (c % OC) * 8;
       }
       const int go = o0 + o, gk = k0 + k;
       v[h] = make_uint4(0, 0, 0, 0);
@@ -90,12 +96,12 @@ struct TileLoad {
   }
   __device__ void store(__bf16* S, int t) const {
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
+    for (int h = 0; h < NCH; h++) {
       const int c = t + h * NT;
       if constexpr (KC) {
         *reinterpret_cast<uint4*>(S + (c >> 3) * LDK + (c & 7) * 8) = v[h];
       } else {
-        const int k = c >> 3, o = (c & 7) * 8;
+        const int k = c / OC, o = (c % OC) * 8;
         __bf16 e[8];
         memcpy(e, &v[h], 16);
 #pragma unroll
@@ -105,22 +111,23 @@ struct TileLoad {
   }
 };
 
-template <bool TA, bool TB, int EPI>
+template <bool TA, bool TB, int EPI, int TM, int TN>
 __global__ void __launch_bounds__(NT) k_gemm(GemmArgs g) {
+  constexpr int MI = TM / 32, NJ = TN / 32;  // 16x16 MFMA tiles per wave
   __shared__ __attribute__((aligned(16))) __bf16 As[TM * LDK];
   __shared__ __attribute__((aligned(16))) __bf16 Bs[TN * LDK];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
-  const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
-  TileLoad<!TA> la;
-  TileLoad<TB> lb;
+  const int wm = (w & 1) * (TM / 2), wn = (w >> 1) * (TN / 2);
+  TileLoad<!TA, TM> la;
+  TileLoad<TB, TN> lb;
   la.load(g.A, g.lda, m0, g.M, 0, g.K, t);
   lb.load(g.B, g.ldb, n0, g.N, 0, g.K, t);
-  f32x4 acc[2][2];
+  f32x4 acc[MI][NJ];
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < MI; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int j = 0; j < NJ; j++) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
   const int r16 = lane & 15, kg = (lane >> 4) * 8;
   for (int k0 = 0; k0 < g.K; k0 += TK) {
     __syncthreads();  // the previous step's MFMAs are done with the tiles
@@ -133,25 +140,25 @@ __global__ void __launch_bounds__(NT) k_gemm(GemmArgs g) {
     }
 #pragma unroll
     for (int kk = 0; kk < TK; kk += 32) {
-      bf16x8 a[2], b[2];
+      bf16x8 a[MI], b[NJ];
 #pragma unroll
-      for (int i = 0; i < 2; i++) a[i] = *reinterpret_cast<const bf16x8*>(As + (wm + i * 16 + r16) * LDK + kk + kg);
+      for (int i = 0; i < MI; i++) a[i] = *reinterpret_cast<const bf16x8*>(As + (wm + i * 16 + r16) * LDK + kk + kg);
 #pragma unroll
-      for (int j = 0; j < 2; j++) b[j] = *reinterpret_cast<const bf16x8*>(Bs + (wn + j * 16 + r16) * LDK + kk + kg);
+      for (int j = 0; j < NJ; j++) b[j] = *reinterpret_cast<const bf16x8*>(Bs + (wn + j * 16 + r16) * LDK + kk + kg);
 #pragma unroll
-      for (int i = 0; i < 2; i++)
+      for (int i = 0; i < MI; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < NJ; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     }
   }
   // epilogue: lane holds rows 4*(lane/16) + r of each 16x16 tile, column lane % 16
 #pragma unroll
-  for (int j = 0; j < 2; j++) {
+  for (int j = 0; j < NJ; j++) {
     const int n = n0 + wn + j * 16 + r16;
     if (n >= g.N) continue;
     const float bias = (EPI == EPI_BIAS_GELU || EPI == EPI_BIAS || EPI == EPI_BIAS_EXP) ? g.bias[n] : 0.0f;
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < MI; i++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
@@ -266,10 +273,18 @@ __global__ void k_adam(float* p, const float* g, float* m, float* v, __bf16* pb,
 
 inline unsigned blocks(long long n, int t) { return (unsigned)((n + t - 1) / t); }
 
+// 64x64 tiles, or 32x32 when the 64x64 grid has fewer than two workgroups
+// per CU (256 CUs).
 template <bool TA, bool TB, int EPI>
 hipError_t gemm(const GemmArgs& g, hipStream_t s) {
-  dim3 grid((g.N + TN - 1) / TN, (g.M + TM - 1) / TM);
-  hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), grid, dim3(NT), 0, s, g);
+  const long long big = (long long)((g.N + 63) / 64) * ((g.M + 63) / 64);
+  if (big >= 512) {
+    dim3 grid((g.N + 63) / 64, (g.M + 63) / 64);
+    hipLaunchKernelGGL((k_gemm<TA, TB, EPI, 64, 64>), grid, dim3(NT), 0, s, g);
+  } else {
+    dim3 grid((g.N + 31) / 32, (g.M + 31) / 32);
+    hipLaunchKernelGGL((k_gemm<TA, TB, EPI, 32, 32>), grid, dim3(NT), 0, s, g);
+  }
   return hipGetLastError();
 }
 

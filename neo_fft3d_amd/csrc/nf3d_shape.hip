@@ -18,8 +18,8 @@
 
 namespace nf3d {
 #if defined(NF3D_PART_TILE)
-hipError_t NF3D_NAME(NF3D_BW, NF3D_BH)(int mode, int bt, int onchip, const TileLaunch& l, hipStream_t s) {
-  return launch_tile_shape<NF3D_BW, NF3D_BH>(mode, bt, onchip, l, s);
+hipError_t NF3D_NAME(NF3D_BW, NF3D_BH)(int mode, int bt, int onchip, int mf, const TileLaunch& l, hipStream_t s) {
+  return launch_tile_shape<NF3D_BW, NF3D_BH>(mode, bt, onchip, mf, l, s);
 }
 #elif defined(NF3D_PART_RUN)
 hipError_t NF3D_RNAME(NF3D_BW, NF3D_BH)(int mode, int bt, int shp, const RunBatch& l, hipStream_t s) {

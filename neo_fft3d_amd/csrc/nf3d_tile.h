@@ -21,6 +21,7 @@
 //      store their partial sums to the seam buffer, finished by k_seam.
 #pragma once
 #include "nf3d_fft.h"
+#include "nf3d_mfma.h"
 #include "nf3d_internal.h"
 
 namespace nf3d {
@@ -468,7 +469,10 @@ __device__ __forceinline__ void overlap_tile(const PlaneJob& J, uint8_t* dst, fl
 }
 
 // ------------------------------------------------------------------ kernel
-template <int BW, int BH, int NT, int MODE, int BT, int ONCHIP>
+// MF: the 2D transforms as DFT-matrix products on the matrix cores (32x32
+// blocks; phases 2-3 and 5-6 become one forward and one inverse product chain
+// per block, a wavefront per block), the rest unchanged.
+template <int BW, int BH, int NT, int MODE, int BT, int ONCHIP, bool MF = false>
 __global__ void __launch_bounds__(NT) k_tile(TileLaunch L_arg) {
   const TileLaunch& L = kernarg_launch(L_arg);
   using S = TShape<BW, BH, NT>;
@@ -517,22 +521,51 @@ __global__ void __launch_bounds__(NT) k_tile(TileLaunch L_arg) {
     else if (pix == 2) gather_tile<2, BW, BH, NT, SA, LA>(J, stage, rowoff, winx, winy, by, bx0, nblk, RW, stepx, t);
     else gather_tile<4, BW, BH, NT, SA, LA>(J, stage, rowoff, winx, winy, by, bx0, nblk, RW, stepx, t);
     __syncthreads();
-    // ---- 2. row r2c, two real rows per lane
-    float2 z[BW];
-    if (row_on) {
-      static_for<0, BW>([&](auto j) { z[j] = stage[rb * LA + rp * SA + decltype(j)::value]; });
-    }
-    __syncthreads();  // gather stage dead; spectral layout from here
-    if (row_on) {
-      fft<BW, -1>(z);
-      float2* dst = spec + rb * LSPEC + 2 * rp * SR;
-      static_for<0, COLS>([&](auto kk) {
-        constexpr int k = decltype(kk)::value;
-        constexpr int km = (BW - k) % BW;
-        const float2 a = z[k], b = z[km];
-        dst[k] = make_float2((a.x + b.x) * 0.5f, (a.y - b.y) * 0.5f);
-        dst[SR + k] = make_float2((a.y + b.y) * 0.5f, (b.x - a.x) * 0.5f);
-      });
+    if constexpr (MF) {
+      // ---- 2-3. 2D r2c on the matrix cores: every block's operands first
+      // (the spectral layout overlaps the gather stage), then the products
+      static_assert(BW == 32 && BH == 32, "matrix-core DFT: 32x32 blocks");
+      constexpr int NW = NT / 64, MB = (BPW + NW - 1) / NW;
+      const int wave = t >> 6, lane = t & 63;
+      const H2* T = static_cast<const H2*>(J.dft_tab);
+      const float sx = J.dft_sx;
+      DftIn din[MB];
+#pragma unroll
+      for (int q = 0; q < MB; q++) {
+        const int b = wave + q * NW;  // uniform in the wavefront
+        if (b < nblk)
+          din[q] = dft32_fwd_load(
+              [&](int r, int c) {
+                const float2 v = stage[b * LA + (r >> 1) * SA + c];
+                return (r & 1) ? v.y : v.x;
+              },
+              sx, lane);
+      }
+      __syncthreads();  // gather stage dead; spectral layout from here
+#pragma unroll
+      for (int q = 0; q < MB; q++) {
+        const int b = wave + q * NW;
+        if (b < nblk)
+          dft32_fwd_compute(din[q], sx, T, lane, [&](int r, int k, float2 zz) { spec[b * LSPEC + r * SR + k] = zz; });
+      }
+    } else {
+      // ---- 2. row r2c, two real rows per lane
+      float2 z[BW];
+      if (row_on) {
+        static_for<0, BW>([&](auto j) { z[j] = stage[rb * LA + rp * SA + decltype(j)::value]; });
+      }
+      __syncthreads();  // gather stage dead; spectral layout from here
+      if (row_on) {
+        fft<BW, -1>(z);
+        float2* dst = spec + rb * LSPEC + 2 * rp * SR;
+        static_for<0, COLS>([&](auto kk) {
+          constexpr int k = decltype(kk)::value;
+          constexpr int km = (BW - k) % BW;
+          const float2 a = z[k], b = z[km];
+          dst[k] = make_float2((a.x + b.x) * 0.5f, (a.y - b.y) * 0.5f);
+          dst[SR + k] = make_float2((a.y + b.y) * 0.5f, (b.x - a.x) * 0.5f);
+        });
+      }
     }
     __syncthreads();
     // ---- 3. column FFT (forward only: store the spectrum and stop)
@@ -540,7 +573,7 @@ __global__ void __launch_bounds__(NT) k_tile(TileLaunch L_arg) {
       if (col_on) {
         float2 col[BH];
         static_for<0, BH>([&](auto i) { col[i] = spec[cb * LSPEC + decltype(i)::value * SR + cc]; });
-        fft<BH, -1>(col);
+        if constexpr (!MF) fft<BH, -1>(col);
         float2* out = J.spec_out + tile_spec0 + (size_t)cb * (BH * COLS) + cc;
         static_for<0, BH>([&](auto i) { out[decltype(i)::value * COLS] = col[i]; });
       }
@@ -565,7 +598,7 @@ __global__ void __launch_bounds__(NT) k_tile(TileLaunch L_arg) {
     if constexpr (PIXELS) {
       if (col_on) {
         static_for<0, BH>([&](auto i) { col[i] = spec[cb * LSPEC + decltype(i)::value * SR + cc]; });
-        fft<BH, -1>(col);
+        if constexpr (!MF) fft<BH, -1>(col);
         if (J.spec_out) {
           float2* out = J.spec_out + colo;
           static_for<0, BH>([&](auto i) { out[decltype(i)::value * COLS] = col[i]; });
@@ -651,10 +684,39 @@ __global__ void __launch_bounds__(NT) k_tile(TileLaunch L_arg) {
         });
       });
       // ---- 5. column inverse FFT, back to LDS for the row c2r
-      fft<BH, +1>(col);
+      if constexpr (!MF) fft<BH, +1>(col);
       static_for<0, BH>([&](auto i) { spec[cb * LSPEC + decltype(i)::value * SR + cc] = col[i]; });
     }
     __syncthreads();
+    if constexpr (MF) {
+      // ---- 5-6. 2D c2r on the matrix cores (DC and Nyquist imaginary parts
+      // dropped by the tables): operands first (the synthesised rows overlap
+      // the spectral layout), then the products
+      constexpr int NW = NT / 64, MB = (BPW + NW - 1) / NW;
+      const int wave = t >> 6, lane = t & 63;
+      const H2* T = static_cast<const H2*>(J.dft_tab);
+      const float sz = J.dft_sz;
+      DftSpec dsp[MB];
+#pragma unroll
+      for (int q = 0; q < MB; q++) {
+        const int b = wave + q * NW;  // uniform in the wavefront
+        if (b < nblk) dsp[q] = dft32_inv_load([&](int r, int k) { return spec[b * LSPEC + r * SR + k]; }, sz, lane);
+      }
+      __syncthreads();  // spectral layout dead; synthesised rows next
+#pragma unroll
+      for (int q = 0; q < MB; q++) {
+        const int b = wave + q * NW;
+        if (b < nblk)
+          dft32_inv_compute(dsp[q], sz, T, lane, [&](int r, int c0, f4 x) {
+            float* d = syn + b * LSYN + r * SYNW + c0;
+            d[0] = x[0];
+            d[1] = x[1];
+            d[2] = x[2];
+            d[3] = x[3];
+          });
+      }
+      __syncthreads();
+    } else {
     // ---- 6. row c2r (Hermitian rows packed in pairs; DC and Nyquist imag dropped)
     {
       float2 z[BW];
@@ -681,6 +743,7 @@ __global__ void __launch_bounds__(NT) k_tile(TileLaunch L_arg) {
         });
       }
       __syncthreads();
+    }
     }
     // ---- 7. overlap-add inside the tile; seam partials to the seam buffer
     if (pix == 1) overlap_tile<1, BW, BH, NT, LSYN, SYNW>(J, J.dst, J.seam, syn, by, bx0, nblk, RW, stepx, stepy, t);
@@ -733,20 +796,20 @@ __global__ void __launch_bounds__(256) k_seam(TileLaunch L_arg) {
   seam_pixel<BW, BH, NT>(J, J.seam, J.dst, yo, x);
 }
 
-template <int BW, int BH>
-hipError_t launch_tile_shape(int mode, int bt, int onchip, const TileLaunch& L, hipStream_t s) {
+template <int BW, int BH, bool MF>
+hipError_t launch_tile_modes(int mode, int bt, int onchip, const TileLaunch& L, hipStream_t s) {
   constexpr int NT = 128;
   dim3 grid(L.total_wg), block(NT);
   switch (mode) {
-    case MODE_FWD: hipLaunchKernelGGL((k_tile<BW, BH, NT, MODE_FWD, 0, 0>), grid, block, 0, s, L); break;
-    case MODE_W2D: hipLaunchKernelGGL((k_tile<BW, BH, NT, MODE_W2D, 0, 0>), grid, block, 0, s, L); break;
-    case MODE_KALMAN: hipLaunchKernelGGL((k_tile<BW, BH, NT, MODE_KALMAN, 0, 0>), grid, block, 0, s, L); break;
-    case MODE_SHARPEN: hipLaunchKernelGGL((k_tile<BW, BH, NT, MODE_SHARPEN, 0, 0>), grid, block, 0, s, L); break;
+    case MODE_FWD: hipLaunchKernelGGL((k_tile<BW, BH, NT, MODE_FWD, 0, 0, MF>), grid, block, 0, s, L); break;
+    case MODE_W2D: hipLaunchKernelGGL((k_tile<BW, BH, NT, MODE_W2D, 0, 0, MF>), grid, block, 0, s, L); break;
+    case MODE_KALMAN: hipLaunchKernelGGL((k_tile<BW, BH, NT, MODE_KALMAN, 0, 0, MF>), grid, block, 0, s, L); break;
+    case MODE_SHARPEN: hipLaunchKernelGGL((k_tile<BW, BH, NT, MODE_SHARPEN, 0, 0, MF>), grid, block, 0, s, L); break;
     case MODE_W3D:
 #define NF3D_W3D(B)                                                                                   \
   case B:                                                                                             \
-    if (onchip) hipLaunchKernelGGL((k_tile<BW, BH, NT, MODE_W3D, B, 1>), grid, block, 0, s, L);       \
-    else hipLaunchKernelGGL((k_tile<BW, BH, NT, MODE_W3D, B, 0>), grid, block, 0, s, L);              \
+    if (onchip) hipLaunchKernelGGL((k_tile<BW, BH, NT, MODE_W3D, B, 1, MF>), grid, block, 0, s, L);   \
+    else hipLaunchKernelGGL((k_tile<BW, BH, NT, MODE_W3D, B, 0, MF>), grid, block, 0, s, L);          \
     break;
       switch (bt) {
         NF3D_W3D(2)
@@ -764,6 +827,18 @@ hipError_t launch_tile_shape(int mode, int bt, int onchip, const TileLaunch& L, 
   dim3 sgrid((L.max_wc + 255) / 256, L.total_seam_rows);
   hipLaunchKernelGGL((k_seam<BW, BH, NT>), sgrid, dim3(256), 0, s, L);
   return hipGetLastError();
+}
+
+// mf: the matrix-core DFT variant (32x32 only)
+template <int BW, int BH>
+hipError_t launch_tile_shape(int mode, int bt, int onchip, int mf, const TileLaunch& L, hipStream_t s) {
+  if (!mf) return launch_tile_modes<BW, BH, false>(mode, bt, onchip, L, s);
+  if constexpr (BW == 32 && BH == 32) {
+    for (int k = 0; k < L.njobs; k++)
+      if (!L.job[k].dft_tab) return hipErrorInvalidValue;
+    return launch_tile_modes<BW, BH, true>(mode, bt, onchip, L, s);
+  }
+  return hipErrorInvalidValue;
 }
 
 template <int BW, int BH>

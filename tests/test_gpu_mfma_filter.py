@@ -1,0 +1,64 @@
+"""The filter with its 2D transforms as DFT-matrix products on the matrix
+cores (FP16 hi/lo split operands, FP32 accumulation; `dft=1`, 32x32 blocks)
+against the FP64 oracle, with the same bars as the FFT form: integer formats
+within +-1 LSB with >= 99.9% of pixels exact, float within 1e-4.  The 10-bit
+4:2:0 cases follow the precision-form workload (bt=3 Wiener and bt=0 Kalman at
+32/16) and report each plane's error against FP64 for both transform forms."""
+import numpy as np
+import pytest
+
+from harness import compare, make_clip
+from test_gpu_parity import check_clip, oracle, product  # noqa: F401  (fixtures)
+
+pytestmark = pytest.mark.gpu
+
+MODES = [
+    "sigma=3;bt=3;bw=32;bh=32",
+    "sigma=3;bt=1;bw=32;bh=32;sharpen=0.5",
+    "sigma=3;bt=0;bw=32;bh=32;ow=16;oh=16;kratio=2",
+    "sigma=3;bt=5;bw=32;bh=32;dehalo=1",
+    "sigma=3;bt=3;bw=32;bh=32;pfactor=1",
+    "sigma=3;bt=-1;bw=32;bh=32;sharpen=1",
+    "sigma=3;bt=2;bw=32;bh=32;ow=16;oh=16;sharpen=0.3;dehalo=0.5",
+    "sigma=3;bt=1;bw=32;bh=32;pfactor=1;pshow=1",
+]
+
+
+@pytest.mark.parametrize("args", MODES)
+def test_gray8_mfma_modes(product, oracle, args):
+    check_clip(product, oracle, "gray", 8, False, args + ";dft=1", W=160, H=128)
+
+
+def test_rgb_float_mfma(product, oracle):
+    check_clip(product, oracle, "rgb", 32, True, "sigma=3;bt=3;bw=32;bh=32;dft=1")
+
+
+def test_yuv_p16_mfma(product, oracle):
+    check_clip(product, oracle, "yuv", 16, False, "sigma=3;bt=3;bw=32;bh=32;ow=16;oh=16;dft=1", exact=0.99)
+
+
+@pytest.mark.parametrize("args", ["sigma=3;bt=3;bw=32;bh=32;ow=16;oh=16",
+                                  "sigma=3;bt=0;kratio=2;bw=32;bh=32;ow=16;oh=16"])
+def test_yuv420p10_precision_form_vs_fp64(product, oracle, args):
+    """Both transform forms against FP64, plane by plane."""
+    clip = make_clip("yuv", 10, False, width=320, height=192, nframes=6, seed=21, noise=12.0)
+    ho = oracle.open(args, clip)
+    want = [ho.get(n) for n in range(6)]
+    for dft in (0, 1):
+        hp = product.open(args + f";dft={dft}", clip)
+        got = [hp.get(n) for n in range(6)]  # once each, in order (Kalman is order dependent)
+        for p in range(3):
+            stats = [compare([got[n][p]], [want[n][p]], False)[0] for n in range(6)]
+            mx = max(s[0] for s in stats)
+            fe = min(s[1] for s in stats)
+            rms = float(np.sqrt(np.mean([s[2] ** 2 for s in stats])))
+            print(f"{args} dft={dft} plane {p}: max {mx:.0f} LSB, exact >= {fe:.5f}, rms {rms:.4f} LSB vs FP64",
+                  flush=True)
+            assert mx <= 1 and fe >= 0.999
+
+
+def test_rejects_other_block_shapes(product):
+    from harness import FilterError
+    clip = make_clip("gray", 8, False, width=160, height=128, nframes=3, seed=2)
+    with pytest.raises(FilterError, match="32x32"):
+        product.open("sigma=3;bw=48;bh=48;dft=1", clip)
