@@ -18,6 +18,8 @@ frame-loop launches through nf3d_run_device_batch).  Prints one JSON line:
   psnr_db       denoised vs clean (noise-free render of the same seed)
   precision_form  the DFT-as-GEMM form (FP16 split operands on the matrix
                 cores) on 3840x2160 32/16 blocks: rate and error vs FP64
+  multi_clip    16 concurrent clips of mixed shape, bt and block size, one
+                batched call per frame step: aggregate fps, step latency
   learned_psd   the learned-PSD form on RGBPS float planes, bt=3, 48/16 with
                 coloured, spatially varying noise: network trained on the GPU,
                 PSNR of plain (best global sigma) vs learned, frames/s of both
@@ -294,6 +296,75 @@ def precision_filter(args):
             "forms_equal_frac": round(float(np.mean([np.mean(x == 0) for x in d])), 6)}
 
 
+def multi_clip(args):
+    """Batched multi-clip serving: 16 concurrent clips of mixed shape -- 4x
+    854x480 YUV420P8, 4x 1280x720 YUV420P8, 4x 1920x1080 YUV420P10, 4x
+    3840x2160 YUV444P16 -- with bt cycling over {0, 1, 3, 5}, bw = bh over
+    {32, 48, 64}, ow = oh the even number nearest bw/3 (10, 16, 22), sigma
+    seeded in [1.5, 4].  One frame step = one nf3d_run_device_batch call with
+    the next frame of every clip (all planes; engines of the same block shape,
+    bt and sharpen case share frame-loop launches).  Reports aggregate fps and
+    the per-step latency; the learned PSD is off (its network is trained for
+    48x48 blocks of float planes)."""
+    import numpy as np
+    import torch
+
+    from neo_fft3d_amd import Engine, default_params, make_format, run_device_batch
+    from neo_fft3d_amd.synth import SEED_BASE, rects_clip
+    F = args.multi_frames
+    shapes = [(854, 480, 8, 1)] * 4 + [(1280, 720, 8, 1)] * 4 + [(1920, 1080, 10, 1)] * 4 + [(3840, 2160, 16, 0)] * 4
+    rng = np.random.default_rng(SEED_BASE + 5)
+    clips = []
+    for i, (w, h, bits, sub) in enumerate(shapes):
+        bt = (0, 1, 3, 5)[i % 4]
+        bw = (32, 48, 64)[i % 3]
+        ow = {32: 10, 48: 16, 64: 22}[bw]
+        sigma = float(rng.uniform(1.5, 4.0))
+        planes = rects_clip(w, h, F, SEED_BASE + 100 + i, ssw=sub, ssh=sub, bits=bits)
+        src = [torch.from_numpy(p).cuda() for p in planes]
+        dst = [torch.zeros_like(x) for x in src]
+        fmt = make_format("yuv", bits, ssw=sub, ssh=sub)
+        es = [Engine(default_params(sigma, bt=bt, bw=bw, bh=bw, ow=ow, oh=ow), fmt, w, h, F, p) for p in range(3)]
+        clips.append(dict(src=src, dst=dst, es=es, w=w, h=h, bits=bits, bt=bt, bw=bw, sigma=round(sigma, 3)))
+    torch.cuda.synchronize()
+
+    def items_for(k):
+        out = []
+        for c in clips:
+            for p, e in enumerate(c["es"]):
+                x, y = c["src"][p], c["dst"][p]
+                st = x[0].stride(0) * x.element_size()
+                out.append((e, k, 1, [x[n].data_ptr() for n in range(F)], st, [y[n].data_ptr() for n in range(F)], st))
+        return out
+    every = [e for c in clips for e in c["es"]]
+
+    def step(k):
+        run_device_batch(items_for(k))
+        for e in every:
+            e.sync()
+    warm = min(4, F // 4)
+    for k in range(warm):
+        step(k)
+    lat = []
+    t0 = time.perf_counter()
+    for k in range(warm, F):
+        t = time.perf_counter()
+        step(k)
+        lat.append((time.perf_counter() - t) * 1e3)
+    total = time.perf_counter() - t0
+    steps = F - warm
+    for c in clips:
+        for e in c["es"]:
+            e.close()
+    return {"workload": "16 clips: 4x 854x480 P8, 4x 1280x720 P8, 4x 1920x1080 P10, 4x 3840x2160 YUV444P16; "
+                        "bt {0,1,3,5}, bw {32,48,64}, ow 10/16/22, sigma U[1.5,4]",
+            "clips": [{k: c[k] for k in ("w", "h", "bits", "bt", "bw", "sigma")} for c in clips],
+            "frame_steps": steps, "fps_aggregate": round(len(clips) * steps / total, 1),
+            "step_latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
+            "step_latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
+            "learned_psd": "off (network trained for 48x48 float planes)"}
+
+
 def learned_form(args):
     """Learned-PSD form on RGBPS float planes, bt=3, 48/16 (coloured noise with
     a Perlin sigma field): the network trained from scratch on the GPU by the
@@ -503,6 +574,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-PSD leg")
     ap.add_argument("--learned-frames", type=int, default=48)
     ap.add_argument("--precision-frames", type=int, default=8)
+    ap.add_argument("--no-multi-clip", action="store_true", help="skip the 16-clip batched leg")
+    ap.add_argument("--multi-frames", type=int, default=24)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run one device copy of MIB MiB first (a reference kernel for traffic counters)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -535,6 +608,8 @@ def main():
         line["precision_form"] = precision_form(args)
     if not args.no_learned:
         line["learned_psd"] = learned_form(args)
+    if not args.no_multi_clip:
+        line["multi_clip"] = multi_clip(args)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args)
         cb = line["cpu_baseline"]["value"]

@@ -966,13 +966,17 @@ struct nf3d_engine {
     }
     if (P.mode == nf3d::MODE_W3D) {
       if (ep.pfactor != 0 && !learned) ensure_pattern3d(ep.bt, s);
-      // the window's older frames of the first step, into slot (frame % bt)
+      // the window's older frames of the first step, into slot (frame % bt),
+      // unless the slot holds that frame already (a previous call's frames)
       for (int k = P.a + P.from; k < P.a + P.to; k++) {
+        if (ring_frame[k % ep.bt] == k) continue;
         PlaneJob F = base_job();
         F.src = static_cast<const uint8_t*>(P.src[k]);
         F.src_pitch = P.src_pitch;
         F.spec_out = slot_ptr(k % ep.bt);
         launch(nf3d::MODE_FWD, 0, F, s);
+        ring_frame[k % ep.bt] = k;
+        ring_tick[k % ep.bt] = ++tick;
       }
     }
   }

@@ -132,6 +132,9 @@ SPLIT_CASES = [
     ("sigma=3;bt=3;bw=16;bh=16;sharpen=0.3", [7, 20, 13]),
     ("sigma=3;bt=5;bw=16;bh=16", [3, 18, 19]),
     ("sigma=3;bt=0;bw=16;bh=16;sharpen=0.3", [1, 22, 17]),
+    # one frame per call (a serving loop): the window's older spectra stay in the ring
+    ("sigma=3;bt=3;bw=48;bh=48;ow=16;oh=16;sharpen=0.3", [1] * 40),
+    ("sigma=3;bt=5;bw=32;bh=32", [2, 1, 1] + [1] * 36),
 ]
 
 
