@@ -157,6 +157,9 @@ int nf3d_mlp_features(nf3d_mlp* m, const void* spec, int n, void* x_out, void* s
 /* FP32 -> BF16 (round to nearest even); |c|^2 of `count` complex values. */
 int nf3d_to_bf16(const float* x, void* out, long long count, void* stream);
 int nf3d_periodogram(const void* spec, float* out, long long count, void* stream);
+/* Training batches: rows idx[0..rows) of src into dst (row_bytes a multiple
+ * of 16; device pointers, int32 row indices). */
+int nf3d_gather_rows(const void* src, const int* idx, void* dst, int rows, int row_bytes, void* stream);
 /* Inference over n rows: out = log-PSD, or exp(log-PSD) * scale when
  * exp_out; `work` holds nf3d_mlp_work_bytes(m, n) bytes. */
 long long nf3d_mlp_work_bytes(const nf3d_mlp* m, int n);

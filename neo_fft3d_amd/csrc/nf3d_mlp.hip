@@ -195,6 +195,14 @@ __global__ void k_features(const float2* spec, __bf16* X, long long total, int d
   X[i] = (__bf16)((logf(p) - mu[k]) * isd[k]);
 }
 
+// Rows idx[r] of src into row r of dst (rows of `words` 16-byte words).
+__global__ void k_gather_rows(const uint4* src, const int* idx, uint4* dst, int rows, int words) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)rows * words) return;
+  const int r = (int)(i / words), w = (int)(i - (long long)r * words);
+  dst[i] = src[(long long)idx[r] * words + w];
+}
+
 __global__ void k_to_bf16(const float* x, __bf16* y, long long total) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < total) y[i] = (__bf16)x[i];
@@ -634,6 +642,15 @@ int nf3d_to_bf16(const float* x, void* out, long long count, void* stream) {
   if (!count) return 0;
   hipLaunchKernelGGL(k_to_bf16, dim3(blocks(count, 256)), dim3(256), 0, st(stream), x, static_cast<__bf16*>(out),
                      count);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int nf3d_gather_rows(const void* src, const int* idx, void* dst, int rows, int row_bytes, void* stream) {
+  if (rows < 0 || row_bytes <= 0 || row_bytes % 16) return -1;
+  if (!rows) return 0;
+  const int words = row_bytes / 16;
+  hipLaunchKernelGGL(k_gather_rows, dim3(blocks((long long)rows * words, 256)), dim3(256), 0, st(stream),
+                     static_cast<const uint4*>(src), idx, static_cast<uint4*>(dst), rows, words);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -54,6 +54,7 @@ def _bind():
     L.nf3d_mlp_features.argtypes = [vp, vp, ip, vp, vp]
     L.nf3d_to_bf16.argtypes = [vp, vp, ctypes.c_longlong, vp]
     L.nf3d_periodogram.argtypes = [vp, vp, ctypes.c_longlong, vp]
+    L.nf3d_gather_rows.argtypes = [vp, vp, vp, ip, ip, vp]
     L.nf3d_mlp_work_bytes.argtypes = [vp, ip]
     L.nf3d_mlp_work_bytes.restype = ctypes.c_longlong
     L.nf3d_mlp_predict.argtypes = [vp, vp, ip, vp, ip, fp, vp, vp, cp, sz]
@@ -357,16 +358,21 @@ def train(mlp: Mlp, data: Dataset, steps: int, batch: int = 4096, lr: float = 1e
     rows = data.rows
     Xb = torch.empty((batch, data.n_in), dtype=torch.bfloat16, device="cuda")
     Tb = torch.empty((batch, data.n_in), dtype=torch.float32, device="cuda")
-    perm = torch.randperm(rows, generator=gen).cuda()
+    L = _bind()
+    perm = torch.randperm(rows, generator=gen).to(torch.int32).cuda()
     at = 0
     for s in range(steps):
         if at + batch > rows:
-            perm = torch.randperm(rows, generator=gen).cuda()
+            perm = torch.randperm(rows, generator=gen).to(torch.int32).cuda()
             at = 0
         idx = perm[at:at + batch]
         at += batch
-        torch.index_select(data.X, 0, idx, out=Xb)
-        torch.index_select(data.T, 0, idx, out=Tb)
+        # the batch's rows, by the library's gather kernel (same stream as the step)
+        if (L.nf3d_gather_rows(ctypes.c_void_p(data.X.data_ptr()), ctypes.c_void_p(idx.data_ptr()),
+                               ctypes.c_void_p(Xb.data_ptr()), batch, data.n_in * 2, None) or
+                L.nf3d_gather_rows(ctypes.c_void_p(data.T.data_ptr()), ctypes.c_void_p(idx.data_ptr()),
+                                   ctypes.c_void_p(Tb.data_ptr()), batch, data.n_in * 4, None)):
+            raise Nf3dError("batch gather failed")
         # cosine decay to lr / 20
         lr_s = lr * (0.05 + 0.95 * 0.5 * (1 + math.cos(math.pi * s / max(steps, 1))))
         want = (s % log_every == 0) or s == steps - 1
