@@ -1233,6 +1233,13 @@ int nf3d_run_device_batch_graph(const nf3d_run_item* items, int n, void* stream,
         for (size_t i = 0; i < prof.size(); i++) items[i].engine->profiling = prof[i];
       }
     } restore{items, prof};
+    // a graph must not depend on what the spectra ring holds when it is
+    // replayed: forget its contents, so the captured work transforms every
+    // frame it reads
+    for (int i = 0; i < n; i++) {
+      nf3d_engine* e = items[i].engine;
+      std::fill(e->ring_frame.begin(), e->ring_frame.end(), -1);
+    }
     check(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
     capturing = true;
     run_batch(items, n, s, false);

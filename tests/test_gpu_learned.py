@@ -415,3 +415,51 @@ def test_learned_path_matches_oracle(torch, family, bits, is_float, bt, per_laun
                 assert mx <= 1 and fe >= 0.999
     for e in engines:
         e.close()
+
+
+def test_graph_replays_with_learned_psd(torch):
+    """A batched call over engines with the learned PSD, captured as a HIP
+    graph: every replay (with eager work between replays) equals the eager
+    call bit for bit -- the per-launch prediction (analysis, features, GEMMs)
+    is part of the graph."""
+    from neo_fft3d_amd import Engine, Graph, default_params, make_format, run_device_batch
+    from neo_fft3d_amd.learned import Mlp, noisy_clip, set_learned
+    F, W, H = 10, 160, 112
+    _, noisy, _ = noisy_clip(W, H, F, seed=44)
+    params = rand_params(D, NH, D, seed=5)
+    params[4] = params[4] * 0.3
+    params[5] = np.full(D, np.log((6.0 / 255.0) ** 2 * 48 * 48), dtype=np.float64)
+    m = Mlp(D, NH, D, 64)
+    m.set_params(mo.pack(params).astype(np.float32))
+    m.set_input_norm(np.full(D, -6.0, np.float32), np.full(D, 0.3, np.float32), 1e-6)
+    fmt = make_format("rgb", 32, True)
+    engs = [Engine(default_params(6.0, bt=3, bw=48, bh=48, ow=16, oh=16), fmt, W, H, F, p) for p in range(3)]
+    for e in engs:
+        set_learned(e, m, 1.0)
+    src = [torch.from_numpy(np.ascontiguousarray(noisy[p])).cuda() for p in range(3)]
+    dst = [torch.zeros_like(x) for x in src]
+    st = W * 4
+    items = [(e, 0, F, [src[p][k].data_ptr() for k in range(F)], st, [dst[p][k].data_ptr() for k in range(F)], st)
+             for p, e in enumerate(engs)]
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with torch.cuda.stream(stream):
+        run_device_batch(items, stream.cuda_stream)
+    stream.synchronize()
+    want = [d.cpu().numpy().copy() for d in dst]
+    g = Graph(items, stream.cuda_stream)
+    for r in range(3):
+        with torch.cuda.stream(stream):
+            for d in dst:
+                d.zero_()
+            x = torch.randn(1 << 20, device="cuda")
+            float(x.sum())  # eager work and a synchronisation between replays
+        g.launch(stream.cuda_stream)
+        stream.synchronize()
+        for p in range(3):
+            got = dst[p].cpu().numpy()
+            print(f"learned graph replay {r} plane {p}: max diff {np.abs(got - want[p]).max():.3g}", flush=True)
+            assert np.array_equal(got, want[p]), (r, p)
+    g.close()
+    for e in engs:
+        e.close()
