@@ -259,15 +259,19 @@ def clean_planes(width: int, height: int, frames: int, seed: int) -> list:
                       noise_chroma=0.0)
 
 
-def noisy_clip(width: int, height: int, frames: int, seed: int):
+def noisy_clip(width: int, height: int, frames: int, seed: int, scale: float = 1.0):
     """(clean, noisy, noise) lists of three [frames, h, w] float32 planes; one
-    noise process per plane."""
+    noise process per plane.  `scale` multiplies content and noise (255 for
+    8-bit sample units, 1023 for 10-bit, ...)."""
     rng = np.random.default_rng(seed)
     clean = clean_planes(width, height, frames, seed)
     noisy, noise = [], []
     for p in range(3):
         model = NoiseModel(height, width, rng)
         n = np.stack([model.sample(rng) for _ in range(frames)])
+        if scale != 1.0:
+            clean[p] = (clean[p] * scale).astype(np.float32)
+            n = (n * scale).astype(np.float32)
         noise.append(n)
         noisy.append((clean[p] + n).astype(np.float32))
     return clean, noisy, noise
@@ -296,15 +300,17 @@ class Dataset:
 
 
 def build_dataset(mlp: Mlp, clips: int, width: int, height: int, frames: int, seed: int, eps: float = 1e-6,
-                  log=None) -> Dataset:
+                  log=None, bw: int = BW, ow: int = OW, scale: float = 1.0) -> Dataset:
     """Blocks of `clips` seeded clips (3 planes x `frames` frames each) through
-    the filter's analysis.  Sets the model's input normalisation from the
-    first clip's features."""
+    the filter's analysis (bw x bw blocks, overlap ow), samples in units of
+    `scale`.  Sets the model's input normalisation from the first clip's
+    features."""
     import torch
-    eng = analysis_engine(width, height)
+    eng = analysis_engine(width, height, bw=bw, ow=ow)
     g = eng.geometry()
     nb = g["nox"] * g["noy"]
-    d = BH * g["outwidth"]
+    d = bw * g["outwidth"]
+    eps = eps * scale * scale
     if d != mlp.n_in or d != mlp.n_out:
         raise ValueError("model width does not match the block shape")
     rows = clips * 3 * frames * nb
@@ -314,7 +320,7 @@ def build_dataset(mlp: Mlp, clips: int, width: int, height: int, frames: int, se
     stride = width * 4
     row = 0
     for c in range(clips):
-        _, noisy, noise = noisy_clip(width, height, frames, seed + c)
+        _, noisy, noise = noisy_clip(width, height, frames, seed + c, scale)
         planes_noisy = torch.from_numpy(np.stack(noisy)).cuda()  # [3][frames][h][w]
         planes_noise = torch.from_numpy(np.stack(noise)).cuda()
         if c == 0:
@@ -386,11 +392,14 @@ def train(mlp: Mlp, data: Dataset, steps: int, batch: int = 4096, lr: float = 1e
 
 
 def train_model(clips: int = 48, frames: int = 2, width: int = 384, height: int = 384, steps: int = 1500,
-                batch: int = 4096, hidden: int = 256, lr: float = 1e-3, seed: int = 0x4C50, log=None):
-    """Model trained from scratch on seeded data; returns (mlp, dataset, losses)."""
-    d = BH * (BW // 2 + 1)
+                batch: int = 4096, hidden: int = 256, lr: float = 1e-3, seed: int = 0x4C50, log=None,
+                bw: int = BW, ow: int = OW, scale: float = 1.0):
+    """Model trained from scratch on seeded data for bw x bw blocks with
+    overlap ow, samples in units of `scale` (1: float planes in [0, 1]; 255:
+    8-bit; 1023: 10-bit; 65535: 16-bit); returns (mlp, dataset, losses)."""
+    d = bw * (bw // 2 + 1)
     mlp = Mlp(d, hidden, d, batch)
-    data = build_dataset(mlp, clips, width, height, frames, seed, log=log)
+    data = build_dataset(mlp, clips, width, height, frames, seed, log=log, bw=bw, ow=ow, scale=scale)
     mlp.set_params(init_params(d, hidden, d, seed, out_bias=data.out_bias))
     losses = train(mlp, data, steps, batch, lr, seed, log=log)
     return mlp, data, losses

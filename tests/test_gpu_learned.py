@@ -463,3 +463,39 @@ def test_graph_replays_with_learned_psd(torch):
     g.close()
     for e in engs:
         e.close()
+
+
+def test_trainer_block_shape_and_sample_scale(torch):
+    """A network for 32x32 blocks (overlap 10) in 8-bit sample units: trained
+    on the GPU, calibrated on a held-out clip -- the mean of ln t - y over its
+    blocks is -0.577 (Euler's gamma) when y = ln E[t]."""
+    from neo_fft3d_amd.learned import analysis_engine, engine_spectra, noisy_clip, train_model
+    mlp, data, losses = train_model(clips=12, frames=2, width=256, height=256, steps=500, bw=32, ow=10,
+                                    scale=255.0)
+    assert mlp.n_in == 32 * 17
+    assert losses[-1][1] < losses[0][1]
+    W = H = 256
+    _, noisy, noise = noisy_clip(W, H, 1, seed=777, scale=255.0)
+    eng = analysis_engine(W, H, bw=32, ow=10)
+    g = eng.geometry()
+    nb = g["nox"] * g["noy"]
+    spec = torch.empty((nb, 544, 2), dtype=torch.float32, device="cuda")
+    xb = torch.empty((nb, 544), dtype=torch.bfloat16, device="cuda")
+    y = torch.empty((nb, 544), dtype=torch.float32, device="cuda")
+    work = torch.empty(mlp.work_bytes(nb), dtype=torch.uint8, device="cuda")
+    r = []
+    for p in range(3):
+        src = torch.from_numpy(np.ascontiguousarray(noisy[p][0])).cuda()
+        nz = torch.from_numpy(np.ascontiguousarray(noise[p][0])).cuda()
+        engine_spectra(eng, src.data_ptr(), W * 4, spec.data_ptr())
+        mlp.features(spec.data_ptr(), nb, xb.data_ptr())
+        mlp.predict(xb.data_ptr(), nb, y.data_ptr(), work.data_ptr())
+        engine_spectra(eng, nz.data_ptr(), W * 4, spec.data_ptr())
+        s = spec.cpu().numpy().astype(np.float64)
+        t = s[..., 0] ** 2 + s[..., 1] ** 2
+        r.append(np.log(t + 1e-30) - y.cpu().numpy().astype(np.float64))
+    r = np.concatenate(r)
+    print(f"32x32 8-bit network: loss {losses[0][1]:.3f} -> {losses[-1][1]:.3f}; held-out mean(ln t - y) "
+          f"{r.mean():.3f} (calibrated: -0.577)", flush=True)
+    assert abs(r.mean() + 0.5772) < 0.3
+    eng.close()
