@@ -63,26 +63,38 @@ __device__ __forceinline__ X& pick(X& a, X& b) {
 }
 
 // ------------------------------------------------------------------ gather
-// Raw samples of one lane's (up to) two tile columns, all BH rows, kept as
-// bits so no use waits on them before the windowing pass.  Both halves are
-// always written (zeros when the tile is one column pass wide) so no old
-// values stay live around the frame loop.
-template <int PIX, int BH>
-__device__ __forceinline__ void gather_issue(const uint8_t* src, const int* rowoff, int sx0, int sx1, bool two,
-                                             uint32_t (&raw)[2][BH]) {
+// Rows handled together by the gather's window read-ahead and the
+// overlap-add's LDS reads: a divisor of the rows of a wavefront (8, else 6, 4
+// or 2), so no chunk runs past them.
+template <int RH>
+constexpr int row_chunk() {
+  return RH % 8 == 0 ? 8 : RH % 6 == 0 ? 6 : RH % 4 == 0 ? 4 : 2;
+}
+
+// Work split of the gather and the overlap-add: each wavefront owns RH = BH /
+// (NT / 64) consecutive tile rows, and walks the tile's columns in passes of
+// 64 (lane l: column l + 64 q), so no wavefront runs a pass in which most of
+// its lanes have no column.
+//
+// Raw samples of one lane's columns (up to NQ passes) in this wavefront's RH
+// rows (rowoff: their source row offsets), kept as bits so no use waits on
+// them before the windowing pass.  Every pass is written (zeros past the
+// tile's passes) so no old values stay live around the frame loop.
+template <int PIX, int RH, int NQ>
+__device__ __forceinline__ void gather_issue(const uint8_t* src, const int* rowoff, const int (&sx)[NQ], int npass,
+                                             uint32_t (&raw)[NQ][RH]) {
   using T = typename std::conditional<PIX == 1, uint8_t, typename std::conditional<PIX == 2, uint16_t, uint32_t>::type>::type;
-  static_for<0, BH>([&](auto rr) {
-    const T* row = reinterpret_cast<const T*>(src + rowoff[decltype(rr)::value]);
-    raw[0][rr] = (uint32_t)row[sx0];
+  static_for<0, NQ>([&](auto qq) {
+    constexpr int q = decltype(qq)::value;
+    if (q == 0 || q < npass) {  // uniform
+      static_for<0, RH>([&](auto rr) {
+        const T* row = reinterpret_cast<const T*>(src + rowoff[decltype(rr)::value]);
+        raw[q][rr] = (uint32_t)row[sx[q]];
+      });
+    } else {
+      static_for<0, RH>([&](auto rr) { raw[q][rr] = 0u; });
+    }
   });
-  if (two) {
-    static_for<0, BH>([&](auto rr) {
-      const T* row = reinterpret_cast<const T*>(src + rowoff[decltype(rr)::value]);
-      raw[1][rr] = (uint32_t)row[sx1];
-    });
-  } else {
-    static_for<0, BH>([&](auto rr) { raw[1][rr] = 0u; });
-  }
 }
 
 // Windowing of the loaded columns into per-block row pairs: the product
@@ -90,17 +102,19 @@ __device__ __forceinline__ void gather_issue(const uint8_t* src, const int* rowo
 // (wy*p)*wx on the other windowed columns of the first/last block rows,
 // (wx*p)*wy elsewhere; src/buffer.cpp:137-378).  A column held by one block
 // writes its sample twice to the same place, so no lane branches per row.
-template <int PIX, int BW, int BH, int NT, int SA, int LA>
-__device__ __forceinline__ void gather_window(const PlaneJob& J, float2* stage, const uint32_t (&raw)[2][BH],
+template <int PIX, int BW, int BH, int NT, int SA, int LA, int RH, int NQ>
+__device__ __forceinline__ void gather_window(const PlaneJob& J, float2* stage, const uint32_t (&raw)[NQ][RH],
                                               const float* winx, const float* winy, int by, int bx0, int nblk,
                                               int RW, int stepx, int t) {
   // launch fields as values: one load each, not one per row
   const int ow = J.ow, oh = J.oh, nox = J.nox, noy = J.noy;
   const float base = J.base;
-  static_for<0, 2>([&](auto pp) {
+  const int l = t & 63, r0 = (t >> 6) * RH;  // lane in the wavefront; its first row
+  constexpr int CW = row_chunk<RH>();         // window rows read ahead together
+  static_for<0, NQ>([&](auto pp) {
     constexpr int p = decltype(pp)::value;
-    if (p == 1 && RW <= NT) return;  // uniform
-    const int c0 = t + p * NT;
+    if (p > 0 && p * 64 >= RW) return;  // uniform
+    const int c0 = l + p * 64;
     const bool cv = c0 < RW;
     const int c = cv ? c0 : RW - 1;
     int bA = c / stepx;
@@ -126,22 +140,23 @@ __device__ __forceinline__ void gather_window(const PlaneJob& J, float2* stage, 
     // windowed column, else 1 ((1*p)*1 = p and (wx*p)*1 = wx*p exactly)
     const float aA = sA != 0 ? wxA : 1.0f, aB = sB != 0 ? wxB : 1.0f;
     const bool top_edge = by == 0, bot_edge = by == noy - 1;  // uniform
-    float wyc[8];  // window values of the current 8 rows, read ahead of their use
-    static_for<0, BH>([&](auto rr) {
-      constexpr int i = decltype(rr)::value;
-      if constexpr (i % 8 == 0) static_for<0, 8>([&](auto uu) { wyc[decltype(uu)::value] = winy[i + decltype(uu)::value]; });
+    float wyc[CW];  // window values of the current CW rows, read ahead of their use
+    static_for<0, RH>([&](auto rr) {
+      constexpr int u = decltype(rr)::value;
+      const int i = r0 + u;  // tile row (uniform)
+      if constexpr (u % CW == 0) static_for<0, CW>([&](auto uu) { wyc[decltype(uu)::value] = winy[i + decltype(uu)::value]; });
       float v;
-      if constexpr (PIX == 4) v = __uint_as_float(raw[p][i]);
-      else v = (float)raw[p][i];
+      if constexpr (PIX == 4) v = __uint_as_float(raw[p][u]);
+      else v = (float)raw[p][u];
       const float px = v - base;
-      constexpr bool ytop = i < (BH / 2), ybot = !ytop;  // which band, if any
-      const bool yw = ytop ? i < oh : i >= BH - oh;     // uniform
-      constexpr int off = (i >> 1) * SA * 2 + (i & 1);
+      const bool ytop = i < (BH / 2);                // which band, if any (uniform)
+      const bool yw = ytop ? i < oh : i >= BH - oh;  // uniform
+      const int off = (i >> 1) * SA * 2 + (u & 1);   // r0 is even
       if (!yw) {
         dA[off] = aA * px;
         dB[off] = aB * px;
       } else {
-        const float wy = wyc[i % 8];
+        const float wy = wyc[u % CW];
         const bool yedge = ytop ? top_edge : bot_edge;  // uniform
         auto windowed = [&](float wx, int side, bool xedge) {
           float a, b;
@@ -184,10 +199,13 @@ __device__ __forceinline__ void overlap_run(const PlaneJob& J, uint8_t* dst, flo
   const int cy0 = by * stepy, cx0 = bx0 * stepx;
   const size_t P = (size_t)J.seam_plane;
   uint8_t* dbase = dst + (long long)J.tc * dpitch + (long long)J.lc * sizeof(T);
-  static_for<0, 2>([&](auto pp) {
+  constexpr int NW = NT / 64, RH = BH / NW, NQ = (TShape<BW, BH, NT>::BPW * BW + 63) / 64;
+  constexpr int OC = row_chunk<RH>();  // rows per chunk
+  const int l = t & 63, r0 = (t >> 6) * RH;  // lane in the wavefront; its first row
+  static_for<0, NQ>([&](auto pp) {
     constexpr int p = decltype(pp)::value;
-    if (p == 1 && RW <= NT) return;  // uniform
-    const int xl = t + p * NT;
+    if (p > 0 && p * 64 >= RW) return;  // uniform
+    const int xl = l + p * 64;
     const int x = cx0 + xl - mirw;
     const bool xin = xl < RW && x >= 0 && x < Wc;
     // x class: 0 exclusive column, 1 band inside the tile, 2 band shared with
@@ -225,18 +243,18 @@ __device__ __forceinline__ void overlap_run(const PlaneJob& J, uint8_t* dst, flo
     const int xc = xin ? x : 0;
     const int pitch_el = (int)(dpitch / (long long)sizeof(T));
     T* dpl = reinterpret_cast<T*>(dbase);
-    // rows in chunks of 8: the chunk's LDS reads are issued together, ahead
-    // of the per-row (uniform) branches
+    // this wavefront's rows in chunks of OC: the chunk's LDS reads are issued
+    // together, ahead of the per-row (uniform) branches
 #pragma unroll 1
-    for (int i0 = 0; i0 < BH; i0 += 8) {
-      float h[8], w[8];
-      static_for<0, 8>([&](auto uu) {
+    for (int i0 = r0; i0 < r0 + RH; i0 += OC) {
+      float h[OC], w[OC];
+      static_for<0, OC>([&](auto uu) {
         constexpr int u = decltype(uu)::value;
         const int i = i0 + u;
         h[u] = syn[la + i * SYNW] * wa + syn[lb + i * SYNW] * wb;
         w[u] = wsyn[i < oh ? YL + i : (i >= stepy ? YR + i - stepy : YL)];
       });
-      static_for<0, 8>([&](auto uu) {
+      static_for<0, OC>([&](auto uu) {
         constexpr int u = decltype(uu)::value;
         const int i = i0 + u;
         const int ry = cy0 + i - mirh;
@@ -369,7 +387,10 @@ __global__ void __launch_bounds__(NT, 2) k_run(RunBatch B_arg) {
   const int nblk = min(BPW, L0.J.nox - bx0);
   const int stepx = BW - L0.J.ow, stepy = BH - L0.J.oh;
   const int RW = (nblk - 1) * stepx + BW;
-  const bool two = RW > NT;
+  // gather / overlap-add split: RH rows per wavefront, up to GQ column passes
+  constexpr int RH = BH / (NT / 64), GQ = (BPW * BW + 63) / 64;
+  static_assert(NT % 64 == 0 && BH % (NT / 64) == 0 && RH % 2 == 0, "row split");
+  const int npass = (RW + 63) >> 6;
 
   {
     const PlaneJob& J = L0.J;
@@ -390,14 +411,20 @@ __global__ void __launch_bounds__(NT, 2) k_run(RunBatch B_arg) {
   }
   __syncthreads();
 
-  // pixels of a frame's on-chip transform, loaded one frame ahead
-  uint32_t raw[2][BH];
+  // pixels of a frame's on-chip transform (this wavefront's rows), loaded
+  // one frame ahead
+  uint32_t raw[GQ][RH];
   auto issue = [&](const PlaneJob& J, const uint8_t* src, const int* ro, int t) {
-    const int sx0 = reflect101(bx0 * stepx + min(t, RW - 1) - J.mirw, J.Wc) + J.lc;
-    const int sx1 = reflect101(bx0 * stepx + min(t + NT, RW - 1) - J.mirw, J.Wc) + J.lc;
-    if (J.pix == 1) gather_issue<1, BH>(src, ro, sx0, sx1, two, raw);
-    else if (J.pix == 2) gather_issue<2, BH>(src, ro, sx0, sx1, two, raw);
-    else gather_issue<4, BH>(src, ro, sx0, sx1, two, raw);
+    const int l = t & 63;
+    int sx[GQ];
+    static_for<0, GQ>([&](auto qq) {
+      constexpr int q = decltype(qq)::value;
+      sx[q] = reflect101(bx0 * stepx + min(l + 64 * q, RW - 1) - J.mirw, J.Wc) + J.lc;
+    });
+    const int* rw = ro + (t >> 6) * RH;
+    if (J.pix == 1) gather_issue<1, RH, GQ>(src, rw, sx, npass, raw);
+    else if (J.pix == 2) gather_issue<2, RH, GQ>(src, rw, sx, npass, raw);
+    else gather_issue<4, RH, GQ>(src, rw, sx, npass, raw);
   };
   issue(L0.J, L0.src[i0], rowoff, (int)threadIdx.x);
 
@@ -434,9 +461,9 @@ __global__ void __launch_bounds__(NT, 2) k_run(RunBatch B_arg) {
     const SpecParams SP(J);
 
     // ---- 1. window the loaded pixels into per-block row pairs
-    if (pix == 1) gather_window<1, BW, BH, NT, SA, LA>(J, stage, raw, wx, wy, by, bx0, nblk, RW, stepx, t);
-    else if (pix == 2) gather_window<2, BW, BH, NT, SA, LA>(J, stage, raw, wx, wy, by, bx0, nblk, RW, stepx, t);
-    else gather_window<4, BW, BH, NT, SA, LA>(J, stage, raw, wx, wy, by, bx0, nblk, RW, stepx, t);
+    if (pix == 1) gather_window<1, BW, BH, NT, SA, LA, RH, GQ>(J, stage, raw, wx, wy, by, bx0, nblk, RW, stepx, t);
+    else if (pix == 2) gather_window<2, BW, BH, NT, SA, LA, RH, GQ>(J, stage, raw, wx, wy, by, bx0, nblk, RW, stepx, t);
+    else gather_window<4, BW, BH, NT, SA, LA, RH, GQ>(J, stage, raw, wx, wy, by, bx0, nblk, RW, stepx, t);
     __syncthreads();
     // ---- 2. row r2c, two real rows per lane
     {
