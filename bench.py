@@ -587,7 +587,7 @@ def main():
     ap.add_argument("--no-precision-form", action="store_true",
                     help="skip the DFT-as-GEMM (MFMA, FP16 split) transform measurement")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-PSD leg")
-    ap.add_argument("--learned-frames", type=int, default=48)
+    ap.add_argument("--learned-frames", type=int, default=96)
     ap.add_argument("--precision-frames", type=int, default=8)
     ap.add_argument("--no-multi-clip", action="store_true", help="skip the 16-clip batched leg")
     ap.add_argument("--multi-frames", type=int, default=24)
