@@ -48,7 +48,12 @@ struct Failure {
 [[noreturn]] void fail(const std::string& m) { throw Failure{m}; }
 
 void check(hipError_t e, const char* what) {
-  if (e != hipSuccess) fail(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+  if (e != hipSuccess) {
+    // clear the runtime's last error, so a later launch check does not
+    // report this failure again (a failed allocation stays "last" otherwise)
+    (void)hipGetLastError();
+    fail(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+  }
 }
 
 void copy_err(char* err, size_t len, const std::string& m) {

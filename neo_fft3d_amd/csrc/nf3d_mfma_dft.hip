@@ -125,6 +125,7 @@ const H2* mfma_dft_table(hipError_t* err) {
     if (e == hipSuccess) e = hipMemcpy(d, h.data(), h.size() * sizeof(H2), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
       hipFree(d);
+      (void)hipGetLastError();  // not left behind for a later launch check
       *err = e;
       return nullptr;
     }

@@ -319,7 +319,10 @@ struct MlpFail {
   std::string msg;
 };
 void mcheck(hipError_t e, const char* what) {
-  if (e != hipSuccess) throw MlpFail{std::string(what) + ": " + hipGetErrorString(e)};
+  if (e != hipSuccess) {
+    (void)hipGetLastError();  // cleared, so later launch checks do not report it again
+    throw MlpFail{std::string(what) + ": " + hipGetErrorString(e)};
+  }
 }
 void merr(char* err, size_t len, const std::string& m) {
   if (err && len) {
