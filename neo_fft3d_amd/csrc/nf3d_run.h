@@ -6,8 +6,8 @@
 // others between frames.  Per frame the phases are those of k_tile (gather,
 // row r2c, column FFT, spectral pass, column inverse, row c2r, overlap-add);
 // what changes is where the data waits are:
-//   - the pixels of the next frame are loaded into registers at the end of
-//     the current frame, in flight across the barrier;
+//   - the pixels of the next frame are loaded into registers before the
+//     current frame's overlap-add, in flight during it and across the barrier;
 //   - the spectral pass runs down each column in chunks of rows: the ring
 //     spectra / Kalman state of chunk k+DR (DR up to 3) and the table rows of
 //     chunk k+1 are loaded while chunk k is computed; the first chunks are
@@ -55,6 +55,9 @@ __device__ __forceinline__ int batch_job(const RunBatch& B, int wg) {
     if (wg >= B.job[k].wg_begin) jb = k;
   return jb;
 }
+
+// Wavefront of a lane index, as a scalar value.
+__device__ __forceinline__ int wave_of(int t) { return __builtin_amdgcn_readfirstlane(t >> 6); }
 
 template <int I, typename X>
 __device__ __forceinline__ X& pick(X& a, X& b) {
@@ -109,8 +112,10 @@ __device__ __forceinline__ void gather_window(const PlaneJob& J, float2* stage, 
   // launch fields as values: one load each, not one per row
   const int ow = J.ow, oh = J.oh, nox = J.nox, noy = J.noy;
   const float base = J.base;
-  const int l = t & 63, r0 = (t >> 6) * RH;  // lane in the wavefront; its first row
-  constexpr int CW = row_chunk<RH>();         // window rows read ahead together
+  // lane in the wavefront; its first row (scalar: the row branches below are
+  // uniform per wavefront and compile to scalar branches, not exec masks)
+  const int l = t & 63, r0 = wave_of(t) * RH;
+  constexpr int CW = row_chunk<RH>();  // window rows read ahead together
   static_for<0, NQ>([&](auto pp) {
     constexpr int p = decltype(pp)::value;
     if (p > 0 && p * 64 >= RW) return;  // uniform
@@ -201,7 +206,7 @@ __device__ __forceinline__ void overlap_run(const PlaneJob& J, uint8_t* dst, flo
   uint8_t* dbase = dst + (long long)J.tc * dpitch + (long long)J.lc * sizeof(T);
   constexpr int NW = NT / 64, RH = BH / NW, NQ = (TShape<BW, BH, NT>::BPW * BW + 63) / 64;
   constexpr int OC = row_chunk<RH>();  // rows per chunk
-  const int l = t & 63, r0 = (t >> 6) * RH;  // lane in the wavefront; its first row
+  const int l = t & 63, r0 = wave_of(t) * RH;  // lane in the wavefront; its first row (scalar)
   static_for<0, NQ>([&](auto pp) {
     constexpr int p = decltype(pp)::value;
     if (p > 0 && p * 64 >= RW) return;  // uniform
@@ -421,7 +426,7 @@ __global__ void __launch_bounds__(NT, 2) k_run(RunBatch B_arg) {
       constexpr int q = decltype(qq)::value;
       sx[q] = reflect101(bx0 * stepx + min(l + 64 * q, RW - 1) - J.mirw, J.Wc) + J.lc;
     });
-    const int* rw = ro + (t >> 6) * RH;
+    const int* rw = ro + wave_of(t) * RH;
     if (J.pix == 1) gather_issue<1, RH, GQ>(src, rw, sx, npass, raw);
     else if (J.pix == 2) gather_issue<2, RH, GQ>(src, rw, sx, npass, raw);
     else gather_issue<4, RH, GQ>(src, rw, sx, npass, raw);
@@ -650,6 +655,11 @@ __global__ void __launch_bounds__(NT, 2) k_run(RunBatch B_arg) {
       }
     }
     __syncthreads();
+    // ---- the next frame's pixels, in flight during the overlap-add and
+    // across the barrier (the registers of the row transforms are dead here;
+    // the last iteration reloads its own frame: an unconditional load keeps
+    // the old values from living through the iteration)
+    issue(J, L.src[min(fi + 1, i1 - 1)], ro, t);
     // ---- 7. overlap-add inside the tile; seam partials to this frame's seam buffer
     uint8_t* dst = L.dst[fi];
     float* seam = J.seam + (size_t)fi * 4 * (size_t)J.seam_plane;
@@ -657,13 +667,19 @@ __global__ void __launch_bounds__(NT, 2) k_run(RunBatch B_arg) {
     if (pix == 1) overlap_run<1, BW, BH, NT, LSYN, SYNW>(J, dst, seam, syn, ws, by, bx0, nblk, RW, stepx, stepy, t);
     else if (pix == 2) overlap_run<2, BW, BH, NT, LSYN, SYNW>(J, dst, seam, syn, ws, by, bx0, nblk, RW, stepx, stepy, t);
     else overlap_run<4, BW, BH, NT, LSYN, SYNW>(J, dst, seam, syn, ws, by, bx0, nblk, RW, stepx, stepy, t);
-    // ---- the next frame's pixels, in flight across the barrier (issued
-    // after the overlap-add's stores, which therefore need no slot or wait
-    // behind them; the last iteration reloads its own frame: an unconditional
-    // load keeps the old values from living through the iteration)
-    issue(J, L.src[min(fi + 1, i1 - 1)], ro, t);
     __syncthreads();  // syn read; the next frame's gather writes the stage
   }
+}
+
+// n / d for 0 <= n < 2^22 and 1 <= d <= 64 (inv = 1/d rounded): the float
+// quotient is within one of the exact one, and one correction each way
+// makes it exact.
+__device__ __forceinline__ int div_small(int n, int d, float inv) {
+  int q = (int)((float)n * inv);
+  const int r = n - q * d;
+  q += r >= d ? 1 : 0;
+  q -= r < 0 ? 1 : 0;
+  return q;
 }
 
 // Seam pixels of every frame of a k_run launch, finished from the partial
@@ -692,27 +708,34 @@ __global__ void __launch_bounds__(256) k_seam_run(RunBatch B_arg) {
   const float* row = sm + (size_t)yo * J.Wc;
   if (i < J.oh && r >= 1) {
     const float wyr = J.wsynyr[i] * J.norm, wyl = J.wsynyl[i] * J.norm;
+    const int Wc = J.Wc, mirw = J.mirw, ow = J.ow;
+    const float inv = 1.0f / (float)stepx;
     constexpr int U = 8;
-    for (int x0 = 0; x0 < J.Wc; x0 += 256 * U) {
-      float up[U], lo[U];
+    for (int x0 = 0; x0 < Wc; x0 += 256 * U) {
+      // every partial of the lane's pixels in flight together: the band
+      // partials always, the corner partials where a vertical seam crosses
+      float up[U], lo[U], cA[U], cB[U];
+      bool cor[U];
 #pragma unroll
       for (int u = 0; u < U; u++) {
-        const int x = min(x0 + u * 256 + (int)threadIdx.x, J.Wc - 1);
+        const int x = min(x0 + u * 256 + (int)threadIdx.x, Wc - 1);
+        const int cx = x + mirw, c = div_small(cx, stepx, inv), j = cx - c * stepx;
+        cor[u] = j < ow && c >= 1 && (c % BPW) == 0;
         up[u] = row[x];
         lo[u] = row[P + x];
+        cA[u] = 0.0f;
+        cB[u] = 0.0f;
+        if (cor[u]) {
+          cA[u] = row[2 * P + x];
+          cB[u] = row[3 * P + x];
+        }
       }
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const int x = x0 + u * 256 + (int)threadIdx.x;
-        if (x >= J.Wc) break;
-        const int cx = x + J.mirw, c = cx / stepx, j = cx - c * stepx;
-        float v;
-        if (j < J.ow && c >= 1 && (c % BPW) == 0) {
-          const float A = up[u], B = lo[u], C = row[2 * P + x], D = row[3 * P + x];
-          v = (A + B) * wyr + (C + D) * wyl + rounder + fbase;
-        } else {
-          v = up[u] + lo[u] + rounder + fbase;
-        }
+        if (x >= Wc) break;
+        const float A = up[u], B = lo[u], C = cA[u], D = cB[u];
+        const float v = cor[u] ? (A + B) * wyr + (C + D) * wyl + rounder + fbase : A + B + rounder + fbase;
         store_px(J, dst, yo, x, v);
       }
     }
