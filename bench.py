@@ -253,8 +253,9 @@ def precision_form(args):
 
 def precision_filter(args):
     """The filter itself in the precision form (dft=1: its 2D transforms as
-    matrix-core DFT products) on YUV420P10 3840x2160, bt=3, 32/16: frames/s
-    of both forms on a resident clip and how often their outputs agree.
+    matrix-core DFT products, in the frame loop) on YUV420P10 3840x2160,
+    bt=3, 32/16: frames/s of both forms on a resident clip and how often their
+    outputs agree.
     (The error of each form against FP64 is measured by the tests.)"""
     import numpy as np
     import torch
@@ -291,7 +292,7 @@ def precision_filter(args):
             e.close()
     d = [np.abs(a.astype(np.int32) - b.astype(np.int32)) for a, b in zip(out[0], out[1])]
     return {"workload": "yuv420p10_3840x2160_bt3_bw32_ow16", "frames": F,
-            "fps_fft_frame_loop": round(fps[0], 1), "fps_mfma_dft_per_frame": round(fps[1], 1),
+            "fps_fft_frame_loop": round(fps[0], 1), "fps_mfma_dft_frame_loop": round(fps[1], 1),
             "forms_max_diff_lsb": int(max(x.max() for x in d)),
             "forms_equal_frac": round(float(np.mean([np.mean(x == 0) for x in d])), 6)}
 

@@ -157,6 +157,15 @@ class Engine:
                                         ctypes.c_void_p(stream), err, 512) != 0:
             raise Nf3dError(err.value.decode())
 
+    def set_dft(self, mode: int):
+        """2D transform form: 0 = FFT in registers, 1 = DFT-matrix products on
+        the matrix cores (32x32 blocks)."""
+        L = lib()
+        L.nf3d_engine_set_dft.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
+        err = ctypes.create_string_buffer(256)
+        if L.nf3d_engine_set_dft(self.h, mode, err, 256) != 0:
+            raise Nf3dError(err.value.decode())
+
     def profile(self, enable: bool):
         lib().nf3d_engine_profile(self.h, int(enable))
 

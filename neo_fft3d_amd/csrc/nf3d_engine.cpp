@@ -922,7 +922,7 @@ struct nf3d_engine {
                       (long long)(tc + Hc) * dst_pitch < (1LL << 31) && 4LL * Hc * Wc < (1LL << 31) &&
                       dst_pitch % pix == 0;
     const bool pshow = ep.pfactor != 0 && ep.pshow;
-    P.run = P.a <= P.b && fits && !pshow && !dft_mode && !(P.mode == nf3d::MODE_SHARPEN && P.shp == nf3d::SHP_NONE);
+    P.run = P.a <= P.b && fits && !pshow && !(P.mode == nf3d::MODE_SHARPEN && P.shp == nf3d::SHP_NONE);
     if (!P.run) return P;
     // launches of equal length (within one frame), two halves of seam
     // buffers in turn
@@ -1099,7 +1099,8 @@ int nf3d_engine_get_frame_device(nf3d_engine* e, int n, const nf3d_frame_ref* fr
 namespace {
 
 // The batch: plans of every item, prologues, the frame-loop launches grouped
-// by (block shape, mode, bt, sharpen case) and launch index, epilogues.
+// by (block shape, mode, bt, sharpen case, transform form) and launch index,
+// epilogues.
 void run_batch(const nf3d_run_item* items, int n, hipStream_t s, bool timed) {
   std::vector<nf3d_engine::RangePlan> plans(n);
   for (int i = 0; i < n; i++) {
@@ -1115,7 +1116,7 @@ void run_batch(const nf3d_run_item* items, int n, hipStream_t s, bool timed) {
     for (int k = i; k < n; k++) {
       nf3d_engine* e = items[k].engine;
       if (!done[k] && plans[k].run && e->ep.bw == e0->ep.bw && e->ep.bh == e0->ep.bh && e->ep.bt == e0->ep.bt &&
-          plans[k].mode == plans[i].mode && plans[k].shp == plans[i].shp) {
+          plans[k].mode == plans[i].mode && plans[k].shp == plans[i].shp && e->dft_mode == e0->dft_mode) {
         group.push_back(k);
         done[k] = true;
       }
@@ -1145,7 +1146,7 @@ void run_batch(const nf3d_run_item* items, int n, hipStream_t s, bool timed) {
         }
         // the seam pass follows on the same stream: run beside the next
         // launch it takes issue slots the latency-bound k_run needs more
-        check(nf3d::launch_run(plans[i].mode, e0->ep.bt, plans[i].shp, e0->ep.bw, e0->ep.bh, B, s),
+        check(nf3d::launch_run(plans[i].mode, e0->ep.bt, plans[i].shp, e0->dft_mode, e0->ep.bw, e0->ep.bh, B, s),
               "frame-loop kernel");
         check(nf3d::launch_seam_run(e0->ep.bw, e0->ep.bh, B, s), "seam kernel");
         for (auto& ev : evs)

@@ -138,7 +138,7 @@ hipError_t launch_pattern(const PatternJob& job, hipStream_t s);
 hipError_t launch_dft_mfma(const void* in, void* out, int nb, int n, int inverse, float scale, hipStream_t s);
 // k_run over the batch's jobs; `shp` is a Shp value.  launch_seam_run
 // finishes the seams of their frames (after the k_run, same stream).
-hipError_t launch_run(int mode, int bt, int shp, int bw, int bh, const RunBatch& b, hipStream_t s);
+hipError_t launch_run(int mode, int bt, int shp, int mf, int bw, int bh, const RunBatch& b, hipStream_t s);
 hipError_t launch_seam_run(int bw, int bh, const RunBatch& b, hipStream_t s);
 bool block_shape_supported(int bw, int bh);
 int tile_blocks_per_wg(int bw, int bh);

@@ -65,7 +65,7 @@ __global__ void __launch_bounds__(256) k_pattern_set(PatternJob J) {
 
 #define X(W, H)                                                                                          \
   hipError_t launch_tile_##W##x##H(int mode, int bt, int onchip, int mf, const TileLaunch& l, hipStream_t s); \
-  hipError_t launch_run_##W##x##H(int mode, int bt, int shp, const RunBatch& l, hipStream_t s); \
+  hipError_t launch_run_##W##x##H(int mode, int bt, int shp, int mf, const RunBatch& l, hipStream_t s); \
   hipError_t launch_seam_run_##W##x##H(const RunBatch& l, hipStream_t s);
 NF3D_SHAPES(X)
 #undef X
@@ -93,8 +93,9 @@ hipError_t launch_tile(int mode, int bt, int onchip, int mf, int bw, int bh, con
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_run(int mode, int bt, int shp, int bw, int bh, const RunBatch& l, hipStream_t s) {
-#define X(W, H) if (bw == W && bh == H) return launch_run_##W##x##H(mode, bt, shp, l, s);
+hipError_t launch_run(int mode, int bt, int shp, int mf, int bw, int bh, const RunBatch& l, hipStream_t s) {
+  if (mf && !mfma_dft_supported(bw, bh)) return hipErrorInvalidValue;
+#define X(W, H) if (bw == W && bh == H) return launch_run_##W##x##H(mode, bt, shp, mf, l, s);
   NF3D_SHAPES(X)
 #undef X
   return hipErrorInvalidValue;

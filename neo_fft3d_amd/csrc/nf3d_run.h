@@ -1,4 +1,4 @@
-// Frame-loop kernel of the denoiser: k_run<BW,BH,NT,MODE,BT,SHP>.
+// Frame-loop kernel of the denoiser: k_run<BW,BH,NT,MODE,BT,SHP,MF>.
 //
 // A workgroup owns one tile (up to BPW horizontally adjacent blocks of one
 // block row, the tile of k_tile) and filters it for a run of consecutive
@@ -19,7 +19,9 @@
 // spectra are the ones the same lanes stored in earlier iterations (or the
 // launches before), so the loop needs no synchronisation beyond the
 // workgroup.  Kalman state likewise stays with its tile.  Seam partials go to
-// a per-frame seam buffer that k_seam_run finishes after the launch.
+// a per-frame seam buffer that k_seam_run finishes after the launch.  With MF
+// the row and column transforms are the matrix-core DFT products of
+// nf3d_mfma.h (32x32 blocks).
 #pragma once
 #include "nf3d_tile.h"
 
@@ -353,8 +355,14 @@ struct TabBuf {
 // the tile), and only stores are predicated: a register written under a lane
 // mask keeps its old value in the other lanes, which would hold it live around
 // the frame loop.
-template <int BW, int BH, int NT, int MODE, int BT, int SHP>
-__global__ void __launch_bounds__(NT, 2) k_run(RunBatch B_arg) {
+//
+// MF: the 2D transforms as DFT-matrix products on the matrix cores (32x32
+// blocks, nf3d_mfma.h), a wavefront per block in turn, in place of the row
+// and column FFTs; the spectral pass is unchanged.  Every block's operands are
+// held at once (the LDS phases overlap), more than 256 registers: one
+// wavefront per SIMD, with no spills.
+template <int BW, int BH, int NT, int MODE, int BT, int SHP, bool MF = false>
+__global__ void __launch_bounds__(NT, MF ? 1 : 2) k_run(RunBatch B_arg) {
   const RunBatch& B0 = kernarg_batch(B_arg);
   const int jb = batch_job(B0, blockIdx.x);
   const RunLaunch& L0 = B0.job[jb];
@@ -470,8 +478,35 @@ __global__ void __launch_bounds__(NT, 2) k_run(RunBatch B_arg) {
     else if (pix == 2) gather_window<2, BW, BH, NT, SA, LA, RH, GQ>(J, stage, raw, wx, wy, by, bx0, nblk, RW, stepx, t);
     else gather_window<4, BW, BH, NT, SA, LA, RH, GQ>(J, stage, raw, wx, wy, by, bx0, nblk, RW, stepx, t);
     __syncthreads();
-    // ---- 2. row r2c, two real rows per lane
-    {
+    if constexpr (MF) {
+      // ---- 2-3. 2D r2c on the matrix cores: every block's operands first
+      // (the spectral layout overlaps the gather stage), then the products
+      static_assert(BW == 32 && BH == 32, "matrix-core DFT: 32x32 blocks");
+      constexpr int NW = NT / 64, MB = (BPW + NW - 1) / NW;
+      const int wave = wave_of(t), lane = t & 63;
+      const H2* T = static_cast<const H2*>(J.dft_tab);
+      const float sx = J.dft_sx;
+      DftIn din[MB];
+#pragma unroll
+      for (int q = 0; q < MB; q++) {
+        const int b = wave + q * NW;  // uniform in the wavefront
+        if (b < nblk)
+          din[q] = dft32_fwd_load(
+              [&](int r, int c) {
+                const float2 v = stage[b * LA + (r >> 1) * SA + c];
+                return (r & 1) ? v.y : v.x;
+              },
+              sx, lane);
+      }
+      __syncthreads();  // gather stage dead; spectral layout from here
+#pragma unroll
+      for (int q = 0; q < MB; q++) {
+        const int b = wave + q * NW;
+        if (b < nblk)
+          dft32_fwd_compute(din[q], sx, T, lane, [&](int r, int k, float2 zz) { spec[b * LSPEC + r * SR + k] = zz; });
+      }
+    } else {
+      // ---- 2. row r2c, two real rows per lane
       float2 z[BW];
       static_for<0, BW>([&](auto j) { z[j] = stage[rb * LA + rp * SA + decltype(j)::value]; });
       __syncthreads();  // gather stage dead; spectral layout from here
@@ -536,7 +571,7 @@ __global__ void __launch_bounds__(NT, 2) k_run(RunBatch B_arg) {
     static_for<0, (DR < NCH ? DR : NCH)>([&](auto kc) { load_ring(kc, rbuf[decltype(kc)::value]); });
     load_tab(std::integral_constant<int, 0>{}, tbuf[0]);
     static_for<0, BH>([&](auto i) { col[i] = spec[cb * LSPEC + decltype(i)::value * SR + cc]; });
-    fft<BH, -1>(col);
+    if constexpr (!MF) fft<BH, -1>(col);
     if constexpr (MODE == MODE_W3D) {
       if (col_on) {
         float2* out = L.ring + (size_t)((m + TO) % BT) * L.spec_elems + colo;
@@ -625,13 +660,40 @@ __global__ void __launch_bounds__(NT, 2) k_run(RunBatch B_arg) {
       });
     }
     // ---- 5. column inverse FFT, back to LDS for the row c2r
-    fft<BH, +1>(col);
+    if constexpr (!MF) fft<BH, +1>(col);
     if (col_on) {
       static_for<0, BH>([&](auto i) { spec[cb * LSPEC + decltype(i)::value * SR + cc] = col[i]; });
     }
     __syncthreads();
-    // ---- 6. row c2r (Hermitian rows packed in pairs; DC and Nyquist imag dropped)
-    {
+    if constexpr (MF) {
+      // ---- 5-6. 2D c2r on the matrix cores (DC and Nyquist imaginary parts
+      // dropped by the tables): operands first (the synthesised rows overlap
+      // the spectral layout), then the products
+      constexpr int NW = NT / 64, MB = (BPW + NW - 1) / NW;
+      const int wave = wave_of(t), lane = t & 63;
+      const H2* T = static_cast<const H2*>(J.dft_tab);
+      const float sz = J.dft_sz;
+      DftSpec dsp[MB];
+#pragma unroll
+      for (int q = 0; q < MB; q++) {
+        const int b = wave + q * NW;  // uniform in the wavefront
+        if (b < nblk) dsp[q] = dft32_inv_load([&](int r, int k) { return spec[b * LSPEC + r * SR + k]; }, sz, lane);
+      }
+      __syncthreads();  // spectral layout dead; synthesised rows next
+#pragma unroll
+      for (int q = 0; q < MB; q++) {
+        const int b = wave + q * NW;
+        if (b < nblk)
+          dft32_inv_compute(dsp[q], sz, T, lane, [&](int r, int c0, f4 x) {
+            float* d = syn + b * LSYN + r * SYNW + c0;
+            d[0] = x[0];
+            d[1] = x[1];
+            d[2] = x[2];
+            d[3] = x[3];
+          });
+      }
+    } else {
+      // ---- 6. row c2r (Hermitian rows packed in pairs; DC and Nyquist imag dropped)
       float2 z[BW];
       const float2* src = spec + rb * LSPEC + 2 * rp * SR;
       static_for<0, COLS>([&](auto kk) {
@@ -752,19 +814,11 @@ __global__ void __launch_bounds__(256) k_seam_run(RunBatch B_arg) {
   }
 }
 
-template <int BW, int BH>
-hipError_t launch_run_shape(int mode, int bt, int shp, const RunBatch& L, hipStream_t s) {
+template <int BW, int BH, bool MF>
+hipError_t launch_run_modes(int mode, int bt, int shp, const RunBatch& L, hipStream_t s) {
   constexpr int NT = 128;
-  if (L.njobs < 1 || L.njobs > RUN_MAX_JOBS) return hipErrorInvalidValue;
-  int wg = 0;
-  for (int k = 0; k < L.njobs; k++) {
-    const RunLaunch& R = L.job[k];
-    if (R.nframes < 1 || R.nframes > RUN_MAX_FRAMES || R.fpw < 1 || R.wg_begin != wg) return hipErrorInvalidValue;
-    wg += R.ntiles * ((R.nframes + R.fpw - 1) / R.fpw);
-  }
-  if (wg != L.total_wg) return hipErrorInvalidValue;
   dim3 grid(L.total_wg), block(NT);
-#define NF3D_RUN(M, B, H) hipLaunchKernelGGL((k_run<BW, BH, NT, M, B, H>), grid, block, 0, s, L)
+#define NF3D_RUN(M, B, H) hipLaunchKernelGGL((k_run<BW, BH, NT, M, B, H, MF>), grid, block, 0, s, L)
 #define NF3D_RUN_SHP(M, B)                                   \
   switch (shp) {                                             \
     case SHP_NONE: NF3D_RUN(M, B, SHP_NONE); break;          \
@@ -794,6 +848,25 @@ hipError_t launch_run_shape(int mode, int bt, int shp, const RunBatch& L, hipStr
 #undef NF3D_RUN_SHP
 #undef NF3D_RUN
   return hipGetLastError();
+}
+
+// mf: the matrix-core DFT form (32x32 blocks only).
+template <int BW, int BH>
+hipError_t launch_run_shape(int mode, int bt, int shp, int mf, const RunBatch& L, hipStream_t s) {
+  if (L.njobs < 1 || L.njobs > RUN_MAX_JOBS) return hipErrorInvalidValue;
+  int wg = 0;
+  for (int k = 0; k < L.njobs; k++) {
+    const RunLaunch& R = L.job[k];
+    if (R.nframes < 1 || R.nframes > RUN_MAX_FRAMES || R.fpw < 1 || R.wg_begin != wg) return hipErrorInvalidValue;
+    if (mf && !R.J.dft_tab) return hipErrorInvalidValue;
+    wg += R.ntiles * ((R.nframes + R.fpw - 1) / R.fpw);
+  }
+  if (wg != L.total_wg) return hipErrorInvalidValue;
+  if (mf) {
+    if constexpr (BW == 32 && BH == 32) return launch_run_modes<BW, BH, true>(mode, bt, shp, L, s);
+    else return hipErrorInvalidValue;
+  }
+  return launch_run_modes<BW, BH, false>(mode, bt, shp, L, s);
 }
 
 template <int BW, int BH>

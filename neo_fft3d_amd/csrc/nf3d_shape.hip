@@ -22,8 +22,8 @@ hipError_t NF3D_NAME(NF3D_BW, NF3D_BH)(int mode, int bt, int onchip, int mf, con
   return launch_tile_shape<NF3D_BW, NF3D_BH>(mode, bt, onchip, mf, l, s);
 }
 #elif defined(NF3D_PART_RUN)
-hipError_t NF3D_RNAME(NF3D_BW, NF3D_BH)(int mode, int bt, int shp, const RunBatch& l, hipStream_t s) {
-  return launch_run_shape<NF3D_BW, NF3D_BH>(mode, bt, shp, l, s);
+hipError_t NF3D_RNAME(NF3D_BW, NF3D_BH)(int mode, int bt, int shp, int mf, const RunBatch& l, hipStream_t s) {
+  return launch_run_shape<NF3D_BW, NF3D_BH>(mode, bt, shp, mf, l, s);
 }
 hipError_t NF3D_SNAME(NF3D_BW, NF3D_BH)(const RunBatch& l, hipStream_t s) {
   return launch_seam_run_shape<NF3D_BW, NF3D_BH>(l, s);

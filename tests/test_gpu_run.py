@@ -50,6 +50,14 @@ CASES = [
     ("sigma=3;bt=3;bw=16;bh=16;sharpen=0.3", "gray", 10, False, 160, 128, 24),
     ("sigma=3;bt=0;bw=32;bh=16;ow=8;oh=4;kratio=1.5", "gray", 16, False, 160, 128, 24),
     ("sigma=3;bt=3;bw=16;bh=16;sharpen=0.3", "rgb", 32, True, 96, 64, 24),
+    # the matrix-core DFT form in the frame loop (32x32 blocks)
+    ("sigma=3;bt=3;bw=32;bh=32;ow=16;oh=16;sharpen=0.3;dft=1", "gray", 8, False, 160, 128, 40),
+    ("sigma=3;bt=5;bw=32;bh=32;dft=1", "gray", 8, False, 160, 128, 24),
+    ("sigma=3;bt=1;bw=32;bh=32;dehalo=1;dft=1", "gray", 8, False, 160, 128, 24),
+    ("sigma=3;bt=0;bw=32;bh=32;ow=16;oh=16;kratio=2;dft=1", "gray", 10, False, 160, 128, 24),
+    ("sigma=3;bt=-1;bw=32;bh=32;sharpen=1;dft=1", "gray", 8, False, 160, 128, 24),
+    ("sigma=3;bt=3;bw=32;bh=32;pfactor=1;dft=1", "gray", 8, False, 160, 128, 24),
+    ("sigma=3;bt=3;bw=32;bh=32;sharpen=0.3;dft=1", "rgb", 32, True, 96, 64, 24),
 ]
 
 INT_FIELDS = {"bw", "bh", "bt", "ow", "oh", "measure", "interlaced", "wintype", "pframe", "px", "py",
@@ -59,10 +67,19 @@ INT_FIELDS = {"bw", "bh", "bt", "ow", "oh", "measure", "interlaced", "wintype", 
 def engine_params(args):
     from neo_fft3d_amd import default_params
     kv = dict(a.split("=") for a in args.split(";"))
+    kv.pop("dft", None)  # an engine setting, not a parameter: make_engine applies it
     p = default_params(float(kv.pop("sigma")))
     for k, v in kv.items():
         setattr(p, k, int(v) if k in INT_FIELDS else float(v))
     return p
+
+
+def make_engine(args, fmt, width, height, frames, plane):
+    from neo_fft3d_amd import Engine
+    e = Engine(engine_params(args), fmt, width, height, frames, plane)
+    if "dft=1" in args.split(";"):
+        e.set_dft(1)
+    return e
 
 
 @pytest.fixture(scope="module")
@@ -84,10 +101,10 @@ def oracle():
 
 def run_range(torch, clip, args, splits):
     """Plane 0 of `clip` through Engine.run_device, called once per split."""
-    from neo_fft3d_amd import Engine, make_format
+    from neo_fft3d_amd import make_format
     F = len(clip.frames)
     fmt = make_format(clip.family, clip.bits, clip.is_float)
-    eng = Engine(engine_params(args), fmt, clip.width, clip.height, F, 0)
+    eng = make_engine(args, fmt, clip.width, clip.height, F, 0)
     host = np.stack([clip.frames[n][0] for n in range(F)])
     src = torch.from_numpy(host).cuda()
     dst = torch.zeros_like(src)
@@ -135,6 +152,8 @@ SPLIT_CASES = [
     # one frame per call (a serving loop): the window's older spectra stay in the ring
     ("sigma=3;bt=3;bw=48;bh=48;ow=16;oh=16;sharpen=0.3", [1] * 40),
     ("sigma=3;bt=5;bw=32;bh=32", [2, 1, 1] + [1] * 36),
+    ("sigma=3;bt=3;bw=32;bh=32;sharpen=0.3;dft=1", [5, 1, 1, 20, 13]),
+    ("sigma=3;bt=0;bw=32;bh=32;dft=1", [1, 22, 17]),
 ]
 
 
@@ -167,18 +186,21 @@ BATCH_CLIPS = [
     ("sigma=3;bt=3;bw=16;bh=16;sharpen=0.3", "gray", 8, 96, 64, 20, 2),
     ("sigma=3;bt=0;bw=16;bh=16", "gray", 8, 128, 96, 24, 3),
     ("sigma=2;bt=3;bw=48;bh=48;ow=16;oh=16;sharpen=0.3", "yuv", 8, 320, 200, 22, 4),
+    # same shape, mode and sharpen case, the two transform forms: separate launches
+    ("sigma=3;bt=3;bw=32;bh=32;sharpen=0.3", "gray", 8, 160, 128, 20, 6),
+    ("sigma=3;bt=3;bw=32;bh=32;sharpen=0.3;dft=1", "gray", 8, 160, 128, 20, 7),
 ]
 
 
 def _batch_setup(torch):
-    from neo_fft3d_amd import Engine, make_format
+    from neo_fft3d_amd import make_format
     jobs = []
     for args, fam, bits, W, H, F, seed in BATCH_CLIPS:
         clip = make_clip(fam, bits, False, width=W, height=H, nframes=F, seed=seed)
         for p in range(clip.nplanes):
             jobs.append((args, clip, p, make_format(fam, bits)))
     def engines():
-        return [Engine(engine_params(a), fmt, c.width, c.height, len(c.frames), p) for a, c, p, fmt in jobs]
+        return [make_engine(a, fmt, c.width, c.height, len(c.frames), p) for a, c, p, fmt in jobs]
     return jobs, engines
 
 
