@@ -522,31 +522,30 @@ __global__ void __launch_bounds__(NT) k_tile(TileLaunch L_arg) {
     else gather_tile<4, BW, BH, NT, SA, LA>(J, stage, rowoff, winx, winy, by, bx0, nblk, RW, stepx, t);
     __syncthreads();
     if constexpr (MF) {
-      // ---- 2-3. 2D r2c on the matrix cores: every block's operands first
-      // (the spectral layout overlaps the gather stage), then the products
+      // ---- 2-3. 2D r2c on the matrix cores, in rounds of one block per
+      // wavefront, highest blocks first: spectral block b overlaps only the
+      // gather stage of blocks b and b+1, so a round writes over stages its own
+      // round or earlier rounds have read (one barrier per round)
       static_assert(BW == 32 && BH == 32, "matrix-core DFT: 32x32 blocks");
+      static_assert(LSPEC >= LA && 2 * LA - LSPEC >= (BPW - 1) * (LSPEC - LA), "spectra overlap stages b, b+1 only");
       constexpr int NW = NT / 64, MB = (BPW + NW - 1) / NW;
-      const int wave = t >> 6, lane = t & 63;
+      const int wave = __builtin_amdgcn_readfirstlane(t >> 6), lane = t & 63;
       const H2* T = static_cast<const H2*>(J.dft_tab);
       const float sx = J.dft_sx;
-      DftIn din[MB];
-#pragma unroll
+#pragma unroll 1
       for (int q = 0; q < MB; q++) {
-        const int b = wave + q * NW;  // uniform in the wavefront
-        if (b < nblk)
-          din[q] = dft32_fwd_load(
+        const int b = nblk - 1 - (q * NW + wave);  // uniform in the wavefront
+        DftIn din;
+        if (b >= 0)
+          din = dft32_fwd_load(
               [&](int r, int c) {
                 const float2 v = stage[b * LA + (r >> 1) * SA + c];
                 return (r & 1) ? v.y : v.x;
               },
               sx, lane);
-      }
-      __syncthreads();  // gather stage dead; spectral layout from here
-#pragma unroll
-      for (int q = 0; q < MB; q++) {
-        const int b = wave + q * NW;
-        if (b < nblk)
-          dft32_fwd_compute(din[q], sx, T, lane, [&](int r, int k, float2 zz) { spec[b * LSPEC + r * SR + k] = zz; });
+        __syncthreads();  // the round's stages read
+        if (b >= 0)
+          dft32_fwd_compute(din, sx, T, lane, [&](int r, int k, float2 zz) { spec[b * LSPEC + r * SR + k] = zz; });
       }
     } else {
       // ---- 2. row r2c, two real rows per lane
@@ -690,24 +689,23 @@ __global__ void __launch_bounds__(NT) k_tile(TileLaunch L_arg) {
     __syncthreads();
     if constexpr (MF) {
       // ---- 5-6. 2D c2r on the matrix cores (DC and Nyquist imaginary parts
-      // dropped by the tables): operands first (the synthesised rows overlap
-      // the spectral layout), then the products
+      // dropped by the tables), in rounds of one block per wavefront, lowest
+      // blocks first: synthesised block b overlaps only the spectra of blocks
+      // b-1 and b
+      static_assert(LSYN <= 2 * LSPEC && 2 * LSPEC - LSYN <= 2 * LSPEC / (BPW > 1 ? BPW - 1 : 1),
+                    "synthesised rows overlap spectra b-1, b only");
       constexpr int NW = NT / 64, MB = (BPW + NW - 1) / NW;
-      const int wave = t >> 6, lane = t & 63;
+      const int wave = __builtin_amdgcn_readfirstlane(t >> 6), lane = t & 63;
       const H2* T = static_cast<const H2*>(J.dft_tab);
       const float sz = J.dft_sz;
-      DftSpec dsp[MB];
-#pragma unroll
+#pragma unroll 1
       for (int q = 0; q < MB; q++) {
-        const int b = wave + q * NW;  // uniform in the wavefront
-        if (b < nblk) dsp[q] = dft32_inv_load([&](int r, int k) { return spec[b * LSPEC + r * SR + k]; }, sz, lane);
-      }
-      __syncthreads();  // spectral layout dead; synthesised rows next
-#pragma unroll
-      for (int q = 0; q < MB; q++) {
-        const int b = wave + q * NW;
+        const int b = q * NW + wave;  // uniform in the wavefront
+        DftSpec dsp;
+        if (b < nblk) dsp = dft32_inv_load([&](int r, int k) { return spec[b * LSPEC + r * SR + k]; }, sz, lane);
+        __syncthreads();  // the round's spectra read
         if (b < nblk)
-          dft32_inv_compute(dsp[q], sz, T, lane, [&](int r, int c0, f4 x) {
+          dft32_inv_compute(dsp, sz, T, lane, [&](int r, int c0, f4 x) {
             float* d = syn + b * LSYN + r * SYNW + c0;
             d[0] = x[0];
             d[1] = x[1];
