@@ -22,6 +22,7 @@
 //   k_adam                Adam update of the FP32 master and its BF16 copy
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -60,6 +61,10 @@ struct GemmArgs {
   const float* Zp;  // pre-activation of the GELU being differentiated
   long long ldz;
   float scale;
+  // split-K (EPI_F32 only): workgroup z sums k in [z*kc, min(K, (z+1)*kc))
+  // into C + z*cz; kc = K and cz = 0 without a split
+  int kc;
+  long long cz;
 };
 
 __device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
@@ -119,24 +124,25 @@ __global__ void __launch_bounds__(NT) k_gemm(GemmArgs g) {
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
   const int wm = (w & 1) * (TM / 2), wn = (w >> 1) * (TN / 2);
+  const int kb = blockIdx.z * g.kc, ke = min(g.K, kb + g.kc);
   TileLoad<!TA, TM> la;
   TileLoad<TB, TN> lb;
-  la.load(g.A, g.lda, m0, g.M, 0, g.K, t);
-  lb.load(g.B, g.ldb, n0, g.N, 0, g.K, t);
+  la.load(g.A, g.lda, m0, g.M, kb, ke, t);
+  lb.load(g.B, g.ldb, n0, g.N, kb, ke, t);
   f32x4 acc[MI][NJ];
 #pragma unroll
   for (int i = 0; i < MI; i++)
 #pragma unroll
     for (int j = 0; j < NJ; j++) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
   const int r16 = lane & 15, kg = (lane >> 4) * 8;
-  for (int k0 = 0; k0 < g.K; k0 += TK) {
+  for (int k0 = kb; k0 < ke; k0 += TK) {
     __syncthreads();  // the previous step's MFMAs are done with the tiles
     la.store(As, t);
     lb.store(Bs, t);
     __syncthreads();
-    if (k0 + TK < g.K) {
-      la.load(g.A, g.lda, m0, g.M, k0 + TK, g.K, t);
-      lb.load(g.B, g.ldb, n0, g.N, k0 + TK, g.K, t);
+    if (k0 + TK < ke) {
+      la.load(g.A, g.lda, m0, g.M, k0 + TK, ke, t);
+      lb.load(g.B, g.ldb, n0, g.N, k0 + TK, ke, t);
     }
 #pragma unroll
     for (int kk = 0; kk < TK; kk += 32) {
@@ -165,7 +171,7 @@ __global__ void __launch_bounds__(NT) k_gemm(GemmArgs g) {
         if (m >= g.M) continue;
         const float v = acc[i][j][r];
         if constexpr (EPI == EPI_F32) {
-          g.C[(long long)m * g.ldc + n] = v;
+          g.C[blockIdx.z * g.cz + (long long)m * g.ldc + n] = v;
         } else if constexpr (EPI == EPI_BIAS_GELU) {
           const float z = v + bias;
           if (g.C) g.C[(long long)m * g.ldc + n] = z;
@@ -281,6 +287,15 @@ __global__ void k_adam(float* p, const float* g, float* m, float* v, __bf16* pb,
 
 inline unsigned blocks(long long n, int t) { return (unsigned)((n + t - 1) / t); }
 
+// out[i] = the S split-K partials of element i added in split order.
+__global__ void k_splitsum(const float* part, int S, long long count, float* out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  float v = part[i];
+  for (int z = 1; z < S; z++) v += part[z * count + i];
+  out[i] = v;
+}
+
 // 64x64 tiles, or 32x32 when the 64x64 grid has fewer than two workgroups
 // per CU (256 CUs).
 template <bool TA, bool TB, int EPI>
@@ -296,6 +311,30 @@ hipError_t gemm(const GemmArgs& g, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Split-K plain FP32 product into C [M][N] (ldc = N): when the 32x32 grid
+// has fewer than four workgroups per CU, K is split in up to SPLIT_MAX
+// slices of at least 512 (multiples of 64), each slice's product goes to the
+// workspace, and k_splitsum adds the slices in order (deterministic).
+constexpr int SPLIT_MAX = 8;
+template <bool TA, bool TB>
+hipError_t gemm_splitk(GemmArgs g, float* ws, hipStream_t s) {
+  const long long wgs = (long long)((g.N + 31) / 32) * ((g.M + 31) / 32);
+  int S = (int)std::min<long long>(SPLIT_MAX, (1024 + wgs - 1) / wgs);
+  S = std::min(S, std::max(1, g.K / 512));
+  if (S <= 1 || wgs >= 1024 || g.ldc != g.N) return gemm<TA, TB, EPI_F32>(g, s);
+  const int kc = ((g.K + S - 1) / S + 63) / 64 * 64;
+  S = (g.K + kc - 1) / kc;
+  float* C = g.C;
+  g.C = ws;
+  g.kc = kc;
+  g.cz = (long long)g.M * g.N;
+  dim3 grid((g.N + 31) / 32, (g.M + 31) / 32, S);
+  hipLaunchKernelGGL((k_gemm<TA, TB, EPI_F32, 32, 32>), grid, dim3(NT), 0, s, g);
+  const long long count = (long long)g.M * g.N;
+  hipLaunchKernelGGL(k_splitsum, dim3(blocks(count, 256)), dim3(256), 0, s, ws, S, count, C);
+  return hipGetLastError();
+}
+
 GemmArgs args(const __bf16* A, long long lda, const __bf16* B, long long ldb, int M, int N, int K) {
   GemmArgs g{};
   g.A = A;
@@ -306,6 +345,8 @@ GemmArgs args(const __bf16* A, long long lda, const __bf16* B, long long ldb, in
   g.N = N;
   g.K = K;
   g.scale = 1.0f;
+  g.kc = K;
+  g.cz = 0;
   return g;
 }
 
@@ -349,6 +390,7 @@ struct nf3d_mlp {
   float eps = 1e-10f;
   // training workspace, max_batch rows
   float *z1 = nullptr, *z2 = nullptr, *y = nullptr, *gf = nullptr, *part = nullptr, *lpart = nullptr;
+  float* wsk = nullptr;  // split-K partials of the weight gradients
   __bf16 *h1 = nullptr, *h2 = nullptr, *d3 = nullptr, *d2 = nullptr, *d1 = nullptr;
   int step = 0;
   std::mutex mu_lock;
@@ -356,7 +398,7 @@ struct nf3d_mlp {
   ~nf3d_mlp() {
     for (void* q : {(void*)p, (void*)g, (void*)m, (void*)v, (void*)pb, (void*)mu, (void*)isd, (void*)z1, (void*)z2,
                     (void*)y, (void*)gf, (void*)part, (void*)lpart, (void*)h1, (void*)h2, (void*)d3, (void*)d2,
-                    (void*)d1})
+                    (void*)d1, (void*)wsk})
       hipFree(q);
   }
 
@@ -402,6 +444,7 @@ struct nf3d_mlp {
     gf = dalloc<float>((size_t)mb * wide);
     part = dalloc<float>((size_t)CS_SLICES * wide);
     lpart = dalloc<float>(blocks((long long)mb * no, 256));
+    wsk = dalloc<float>((size_t)SPLIT_MAX * std::max({(size_t)ni * h, (size_t)h * h, (size_t)h * no}));
     h1 = dalloc<__bf16>((size_t)mb * h);
     h2 = dalloc<__bf16>((size_t)mb * h);
     d3 = dalloc<__bf16>((size_t)mb * no);
@@ -472,7 +515,7 @@ struct nf3d_mlp {
     a = args(h2, nh, d3, n_out, nh, n_out, n);
     a.C = g + o_w3;
     a.ldc = n_out;
-    mcheck(gemm<true, false, EPI_F32>(a, s), "dW3");
+    mcheck(gemm_splitk<true, false>(a, wsk, s), "dW3");
     colsum(gf, n, n_out, g + o_b3, s);
     a = args(d3, n_out, pb + o_w3, n_out, n, nh, n_out);
     a.H = d2;
@@ -486,7 +529,7 @@ struct nf3d_mlp {
     a = args(h1, nh, d2, nh, nh, nh, n);
     a.C = g + o_w2;
     a.ldc = nh;
-    mcheck(gemm<true, false, EPI_F32>(a, s), "dW2");
+    mcheck(gemm_splitk<true, false>(a, wsk, s), "dW2");
     colsum(gf, n, nh, g + o_b2, s);
     a = args(d2, nh, pb + o_w2, nh, n, nh, nh);
     a.H = d1;
@@ -500,7 +543,7 @@ struct nf3d_mlp {
     a = args(X, n_in, d1, nh, n_in, nh, n);
     a.C = g + o_w1;
     a.ldc = nh;
-    mcheck(gemm<true, false, EPI_F32>(a, s), "dW1");
+    mcheck(gemm_splitk<true, false>(a, wsk, s), "dW1");
     colsum(gf, n, nh, g + o_b1, s);
   }
 
