@@ -232,14 +232,16 @@ def test_batch_matches_separate_calls(torch_cuda):
         assert np.array_equal(got, want[k]), k
 
 
-def test_graph_replays_match_eager(torch_cuda):
-    """The batch captured as a HIP graph: every replay equals the eager call."""
-    from neo_fft3d_amd import Engine, Graph, make_format, run_device_batch
+@pytest.mark.parametrize("args", ["sigma=2;bt=3;bw=48;bh=48;ow=16;oh=16;sharpen=0.3",
+                                  "sigma=2;bt=3;bw=32;bh=32;ow=16;oh=16;sharpen=0.3;dft=1"])
+def test_graph_replays_match_eager(torch_cuda, args):
+    """The batch captured as a HIP graph: every replay equals the eager call
+    (FFT form, and the matrix-core DFT form)."""
+    from neo_fft3d_amd import Graph, make_format, run_device_batch
     torch = torch_cuda
-    args = "sigma=2;bt=3;bw=48;bh=48;ow=16;oh=16;sharpen=0.3"
     clip = make_clip("yuv", 8, False, width=320, height=200, nframes=22, seed=9)
     fmt = make_format("yuv", 8)
-    engs = [Engine(engine_params(args), fmt, clip.width, clip.height, 22, p) for p in range(3)]
+    engs = [make_engine(args, fmt, clip.width, clip.height, 22, p) for p in range(3)]
     bufs = [_device_clip(torch, clip, p) for p in range(3)]
     items = [(e, 0, 22, sp, st, dp, st) for e, (src, dst, sp, dp, st) in zip(engs, bufs)]
     stream = torch.cuda.Stream()
