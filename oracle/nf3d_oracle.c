@@ -923,3 +923,17 @@ int nf3d_oracle_geometry(void* h, int plane, int* out5) {
   out5[0] = e->nox; out5[1] = e->noy; out5[2] = e->outwidth; out5[3] = e->mirw; out5[4] = e->mirh;
   return 0;
 }
+
+/* The plane's noise pattern (pattern2d, [bh*outwidth]) and the block it was
+ * taken from, for tests; -1 while no pattern is set.  No reference
+ * counterpart (the reference keeps them private in FFT3DEngine). */
+int nf3d_oracle_pattern(void* h, int plane, double* out, int* pxpy) {
+  oclip* c = (oclip*)h;
+  if (!c->has_eng[plane]) return -1;
+  oengine* e = &c->eng[plane];
+  if (!e->pattern_set) return -1;
+  memcpy(out, e->pattern2d, (size_t)e->ep.bh * e->outwidth * sizeof(double));
+  pxpy[0] = e->ep.px;
+  pxpy[1] = e->ep.py;
+  return 0;
+}

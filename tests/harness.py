@@ -16,7 +16,9 @@ from dataclasses import dataclass
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_SO = os.path.join(ROOT, "oracle", "libnf3d_oracle.so")
+# NF3D_ORACLE_SO points the tests at another build of the oracle (the
+# planted-mistake checks of tests/test_oracle_mutations.py use it)
+ORACLE_SO = os.environ.get("NF3D_ORACLE_SO") or os.path.join(ROOT, "oracle", "libnf3d_oracle.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libnf3d_ref.so")
 
 FAMILY = {"gray": 0, "yuv": 1, "rgb": 2}
