@@ -19,6 +19,15 @@
  * it raises; the caller prefixes "FFT3D: " / "neo_fft3d: " as the bridge
  * does).  Host pointers are borrowed for the duration of a call; device
  * memory is owned by the library.  Plain C types only.
+ *
+ * Streams: calls that take a stream enqueue their work on it (NULL: the
+ * engine's own non-blocking stream, which does not wait for work the caller
+ * queued on other streams -- the caller orders its producers before the
+ * call).  Successive calls on one engine are ordered by the library: when a
+ * call uses another stream than the engine's previous call, its stream first
+ * waits for that call's work (the spectra ring, Kalman state and seam buffers
+ * carry over between calls).  Graph replays count as calls on the replay
+ * stream.
  */
 #ifndef NF3D_H
 #define NF3D_H
@@ -192,8 +201,9 @@ int nf3d_engine_set_learned(nf3d_engine* e, nf3d_mlp* m, float strength, char* e
 /* Precision form of the engine's 2D transforms (no reference counterpart; the
  * reference uses FFTW): mode 0 = FP32 FFT in registers (default), 1 = dense
  * DFT-matrix products on the matrix cores with FP16 hi/lo split operands and
- * FP32 accumulation (32x32 blocks; frames then go through the per-frame
- * kernel).  Errors: other block shapes, other modes. */
+ * FP32 accumulation (32x32 blocks, integer formats: the FP16 operand scales
+ * come from the format's sample range).  Errors: other block shapes, float
+ * formats, other modes. */
 int nf3d_engine_set_dft(nf3d_engine* e, int mode, char* err, size_t errlen);
 
 /* ------------------------------------------------------------ launch tuning
@@ -205,7 +215,10 @@ int nf3d_engine_set_dft(nf3d_engine* e, int mode, char* err, size_t errlen);
  * A process-wide table, one line per "bw bh bits bt frames_per_launch
  * frames_per_workgroup" ('#' comments; bits 32 = float), is read from the file
  * named by NF3D_TUNE_TABLE or by nf3d_tune_load (which replaces it and returns
- * its line count); engines created afterwards take their entry.
+ * its line count); engines created afterwards take their entry, except that
+ * attaching a learned PSD (whose prediction is made once per launch, so that
+ * frames per launch is part of its result) restores the default 16 frames
+ * per launch unless nf3d_engine_set_tuning set it.
  * neo_fft3d_amd/autotune.py measures the choices and writes such a table. */
 int nf3d_tune_load(const char* path, char* err, size_t errlen);
 int nf3d_tune_lookup(int bw, int bh, int bits, int bt, int* out2);  /* 1 if found */

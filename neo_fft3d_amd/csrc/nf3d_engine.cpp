@@ -64,9 +64,10 @@ void copy_err(char* err, size_t len, const std::string& m) {
   }
 }
 
-// Launch tuning of the frame-loop path (results do not depend on it): frames
-// per k_run launch and, for the modes without a temporal recursion (Wiener
-// 2D, sharpen), frames per workgroup.  A process-wide table keyed by block
+// Launch tuning of the frame-loop path (results do not depend on it, except
+// with a learned PSD, which is predicted once per launch: table entries do
+// not apply to such engines): frames per k_run launch and, for the modes
+// without a temporal recursion (Wiener 2D, sharpen), frames per workgroup.  A process-wide table keyed by block
 // shape, bit depth and bt, loaded by nf3d_tune_load or from the file named
 // by NF3D_TUNE_TABLE; engines take their entry when created.
 struct TuneEntry {
@@ -165,6 +166,12 @@ struct nf3d_engine {
   int stage_next = 0;
   uint8_t* d_dst = nullptr;
   std::mutex mu;
+  // Ordering between calls on different streams: the work of the last call
+  // ends with done_ev on done_stream, and a call on another stream waits for
+  // it first (the spectra ring, the Kalman state and the seam and staging
+  // buffers carry over from one call to the next).
+  hipEvent_t done_ev = nullptr;
+  hipStream_t done_stream = nullptr;
   // opt-in HIP-event timing of the main spectral kernel of each frame
   bool profiling = false;
   std::vector<hipEvent_t> ev_pool;
@@ -177,8 +184,9 @@ struct nf3d_engine {
   static constexpr int kRunFrames = nf3d::RUN_MAX_FRAMES;  // largest k_run launch (frames)
   int run_frames = kRunFrames;  // frames per k_run launch (tuning)
   int run_fpw = 1;              // frames per workgroup, Wiener 2D / sharpen (tuning)
+  bool run_frames_from_table = false;  // run_frames came from the tuning table, not the caller
   // 2D transforms: 0 = FFT in registers, 1 = DFT-matrix products on the matrix
-  // cores (32x32 blocks; every frame through the per-frame kernel)
+  // cores (32x32 blocks; in both the frame-loop and the per-frame kernel)
   int dft_mode = 0;
   const void* dft_tab = nullptr;
   float dft_sx = 1.0f, dft_sz = 1.0f;
@@ -211,6 +219,7 @@ struct nf3d_engine {
     for (auto* p : d_stage) hipFree(p);
     hipFree(d_dst);
     for (auto ev : ev_pool) hipEventDestroy(ev);
+    if (done_ev) hipEventDestroy(done_ev);
     if (stream) hipStreamDestroy(stream);
   }
 
@@ -462,6 +471,7 @@ struct nf3d_engine {
     if (tune_lookup(bw, bh, fmt.is_float ? 32 : fmt.bits_per_sample, ep.bt, &tune)) {
       run_frames = tune.frames;
       run_fpw = tune.fpw;
+      run_frames_from_table = true;
     }
     if (ep.bt >= 2) {
       ring_slots = ep.bt + 2;
@@ -589,6 +599,18 @@ struct nf3d_engine {
     hipEvent_t a = ev_pool[ev_used++], b = ev_pool[ev_used++];
     check(hipEventRecord(a, s), "hipEventRecord");
     return {a, b};
+  }
+
+  // A call on stream s starts after the previous call's work (if that went
+  // to another stream) ...
+  void order_before(hipStream_t s) {
+    if (done_stream && done_stream != s) check(hipStreamWaitEvent(s, done_ev, 0), "hipStreamWaitEvent");
+  }
+  // ... and marks its own end.
+  void order_after(hipStream_t s) {
+    if (!done_ev) check(hipEventCreateWithFlags(&done_ev, hipEventDisableTiming), "hipEventCreate");
+    check(hipEventRecord(done_ev, s), "hipEventRecord");
+    done_stream = s;
   }
 
   void launch(int mode, int bt, const PlaneJob& J, hipStream_t s, bool timed = false, int onchip = 0) {
@@ -1002,6 +1024,8 @@ struct nf3d_engine {
 
   void get_frame_host(int n, const nf3d_frame_ref* frames_, int nframes, void* dst, int dst_stride) {
     Call c{frames_, nframes, false, stream};
+    order_before(stream);
+    done_stream = nullptr;  // this call waits for its own work below
     std::fill(stage_frame.begin(), stage_frame.end(), -1);
     size_t dpitch = ((size_t)framewidth * pix + 255) / 256 * 256;
     if (!d_dst) d_dst = dev_alloc<uint8_t>(dpitch * frameheight);
@@ -1088,7 +1112,9 @@ int nf3d_engine_get_frame_device(nf3d_engine* e, int n, const nf3d_frame_ref* fr
   std::lock_guard<std::mutex> lock(e->mu);
   try {
     nf3d_engine::Call c{frames, nframes, true, stream ? static_cast<hipStream_t>(stream) : e->stream};
+    e->order_before(c.s);
     e->run(n, c, static_cast<uint8_t*>(dst), dst_stride);
+    e->order_after(c.s);
   } catch (const Failure& f) {
     copy_err(err, errlen, f.msg);
     return -1;
@@ -1101,7 +1127,9 @@ namespace {
 // The batch: plans of every item, prologues, the frame-loop launches grouped
 // by (block shape, mode, bt, sharpen case, transform form) and launch index,
 // epilogues.
-void run_batch(const nf3d_run_item* items, int n, hipStream_t s, bool timed) {
+void run_batch(const nf3d_run_item* items, int n, hipStream_t s, bool timed, bool capture = false) {
+  if (!capture)
+    for (int i = 0; i < n; i++) items[i].engine->order_before(s);
   std::vector<nf3d_engine::RangePlan> plans(n);
   for (int i = 0; i < n; i++) {
     const nf3d_run_item& it = items[i];
@@ -1155,6 +1183,8 @@ void run_batch(const nf3d_run_item* items, int n, hipStream_t s, bool timed) {
     }
   }
   for (int i = 0; i < n; i++) items[i].engine->run_epilogue(plans[i], s);
+  if (!capture)
+    for (int i = 0; i < n; i++) items[i].engine->order_after(s);
 }
 
 // Locks every engine of a batch (address order), after checking the items.
@@ -1193,9 +1223,16 @@ int nf3d_engine_run_device(nf3d_engine* e, int first, int count, const void* con
 }
 
 
+// A captured batch.  The capture runs nothing, but the batch's bookkeeping
+// records which frames its launches leave in the spectra ring; those records
+// are kept here (per engine) and given to the engines when a replay has
+// actually run, while after the capture itself the engines' records say the
+// ring holds nothing known.
 struct nf3d_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  std::vector<nf3d_engine*> engines;
+  std::vector<std::vector<int>> ring_after;  // ring_frame of each engine after one replay
 };
 
 int nf3d_run_device_batch(const nf3d_run_item* items, int n, void* stream, char* err, size_t errlen) {
@@ -1220,8 +1257,14 @@ int nf3d_run_device_batch_graph(const nf3d_run_item* items, int n, void* stream,
     auto locks = lock_batch(items, n);
     for (int i = 0; i < n; i++) {
       nf3d_engine* e = items[i].engine;
-      if (e->ep.pfactor != 0 && (!e->pattern_set || (e->ep.bt >= 2 && !e->pattern3d_init)))
+      if (e->ep.pfactor != 0 && e->ep.pshow) {
+        if (!e->d_ones)
+          fail("the pattern display allocates its buffers on its first call, which cannot be captured: run the "
+               "batch once first");
+      } else if (e->ep.pfactor != 0 && !e->learned &&
+                 (!e->pattern_set || (e->ep.bt >= 2 && !e->pattern3d_init))) {
         fail("the noise pattern is estimated on the first call, which cannot be captured: run the batch once first");
+      }
       // buffers the launches use, allocated outside the capture
       e->plan_range(items[i].first, items[i].count, items[i].src, items[i].src_stride, items[i].dst,
                     items[i].dst_stride);
@@ -1248,9 +1291,16 @@ int nf3d_run_device_batch_graph(const nf3d_run_item* items, int n, void* stream,
     }
     check(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
     capturing = true;
-    run_batch(items, n, s, false);
+    run_batch(items, n, s, false, true);
     auto* g = new nf3d_graph;
     capturing = false;
+    // what a replay leaves in each ring; nothing has run yet
+    for (int i = 0; i < n; i++) {
+      nf3d_engine* e = items[i].engine;
+      g->engines.push_back(e);
+      g->ring_after.push_back(e->ring_frame);
+      std::fill(e->ring_frame.begin(), e->ring_frame.end(), -1);
+    }
     hipError_t e1 = hipStreamEndCapture(s, &g->graph);
     if (e1 != hipSuccess) {
       delete g;
@@ -1276,7 +1326,21 @@ int nf3d_run_device_batch_graph(const nf3d_run_item* items, int n, void* stream,
 
 int nf3d_graph_launch(nf3d_graph* g, void* stream) {
   if (!g || !g->exec) return -1;
-  return hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)) == hipSuccess ? 0 : -1;
+  std::vector<nf3d_engine*> es = g->engines;
+  std::sort(es.begin(), es.end());
+  std::vector<std::unique_lock<std::mutex>> locks;
+  for (nf3d_engine* e : es) locks.emplace_back(e->mu);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  try {
+    for (nf3d_engine* e : g->engines) e->order_before(s);
+    check(hipGraphLaunch(g->exec, s), "hipGraphLaunch");
+    for (nf3d_engine* e : g->engines) e->order_after(s);
+  } catch (const Failure&) {
+    return -1;
+  }
+  // the replay leaves the frames recorded at capture in the rings
+  for (size_t i = 0; i < g->engines.size(); i++) g->engines[i]->ring_frame = g->ring_after[i];
+  return 0;
 }
 
 void nf3d_graph_destroy(nf3d_graph* g) {
@@ -1302,7 +1366,10 @@ int nf3d_engine_spectra(nf3d_engine* e, const void* src, int src_stride, void* s
     F.src = static_cast<const uint8_t*>(src);
     F.src_pitch = src_stride;
     F.spec_out = static_cast<float2*>(spec_out);
-    e->launch(nf3d::MODE_FWD, 0, F, stream ? static_cast<hipStream_t>(stream) : e->stream);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : e->stream;
+    e->order_before(s);
+    e->launch(nf3d::MODE_FWD, 0, F, s);
+    e->order_after(s);
   } catch (const Failure& f) {
     copy_err(err, errlen, f.msg);
     return -1;
@@ -1334,6 +1401,13 @@ int nf3d_engine_set_learned(nf3d_engine* e, nf3d_mlp* m, float strength, char* e
     e->d_lwork = dev_alloc<uint8_t>((size_t)nf3d_mlp_work_bytes(m, e->nb));
     e->learned = m;
     e->learned_strength = strength;
+    // the learned PSD is predicted once per frame-loop launch, so frames per
+    // launch is part of its result: a tuning-table entry (chosen for speed)
+    // does not apply to it; only nf3d_engine_set_tuning changes it
+    if (e->run_frames_from_table) {
+      e->run_frames = nf3d_engine::kRunFrames;
+      e->run_frames_from_table = false;
+    }
   } catch (const Failure& f) {
     copy_err(err, errlen, f.msg);
     return -1;
@@ -1347,6 +1421,9 @@ int nf3d_engine_set_dft(nf3d_engine* e, int mode, char* err, size_t errlen) {
     if (mode != 0 && mode != 1) fail("dft mode must be 0 (FFT) or 1 (matrix-core DFT)");
     if (mode == 1 && !nf3d::mfma_dft_supported(e->ep.bw, e->ep.bh))
       fail("the matrix-core DFT supports 32x32 blocks");
+    // operands are scaled to FP16 from the format's range, which float
+    // samples do not bound
+    if (mode == 1 && e->fmt.is_float) fail("the matrix-core DFT supports integer formats only");
     check(hipSetDevice(e->device), "hipSetDevice");
     if (mode == 1 && !e->dft_tab) {
       hipError_t he;
@@ -1403,6 +1480,7 @@ int nf3d_engine_set_tuning(nf3d_engine* e, int frames_per_launch, int frames_per
   std::lock_guard<std::mutex> lock(e->mu);
   e->run_frames = frames_per_launch;
   e->run_fpw = frames_per_workgroup;
+  e->run_frames_from_table = false;
   return 0;
 }
 
@@ -1447,7 +1525,12 @@ int nf3d_engine_profile_read(nf3d_engine* e, double* total_ms, int* frames) {
   return 0;
 }
 
-int nf3d_engine_sync(nf3d_engine* e) { return hipStreamSynchronize(e->stream) == hipSuccess ? 0 : -1; }
+int nf3d_engine_sync(nf3d_engine* e) {
+  std::lock_guard<std::mutex> lock(e->mu);
+  if (hipStreamSynchronize(e->stream) != hipSuccess) return -1;
+  if (e->done_ev && hipEventSynchronize(e->done_ev) != hipSuccess) return -1;
+  return 0;
+}
 
 void nf3d_engine_destroy(nf3d_engine* e) { delete e; }
 
