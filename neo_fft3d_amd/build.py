@@ -48,7 +48,20 @@ def _run(cmd):
     return r.stdout
 
 
-def build_product(verbose: bool = False) -> str:
+def _stale(obj: str, src: str) -> bool:
+    """An object is stale when older than its source or any header of csrc/
+    (every unit may include any of them), or, for host units, of include/."""
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    deps = [src] + [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
+    if src.endswith(".cpp"):
+        deps += [os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROOT, "include"))]
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def build_product(verbose: bool = False, incremental: bool = False) -> str:
+    """Compiles every unit (incremental: only units older than their sources)."""
     os.makedirs(BUILD, exist_ok=True)
     jobs = []
     objs = []
@@ -68,7 +81,9 @@ def build_product(verbose: bool = False) -> str:
         o = os.path.join(BUILD, u + ".o")
         jobs.append([HIPCC, *HIP_FLAGS, "-x", "hip", "-c", os.path.join(CSRC, u), "-o", o])
         objs.append(o)
-    with ThreadPoolExecutor(max_workers=min(os.cpu_count() or 8, 16, len(jobs))) as ex:
+    if incremental:
+        jobs = [j for j in jobs if _stale(j[-1], j[-3])]
+    with ThreadPoolExecutor(max_workers=max(1, min(os.cpu_count() or 8, 16, len(jobs)))) as ex:
         for out in ex.map(_run, jobs):
             if verbose and out.strip():
                 print(out)
@@ -88,9 +103,10 @@ def build_synth() -> str:
 
 
 def main(argv=None) -> int:
-    verbose = "-v" in (argv or sys.argv[1:])
+    args = argv or sys.argv[1:]
+    verbose = "-v" in args
     build_synth()
-    lib = build_product(verbose)
+    lib = build_product(verbose, incremental="--incremental" in args)
     print(f"built {lib}")
     return 0
 
