@@ -206,7 +206,7 @@ __device__ __forceinline__ void overlap_run(const PlaneJob& J, uint8_t* dst, flo
   const int cy0 = by * stepy, cx0 = bx0 * stepx;
   const size_t P = (size_t)J.seam_plane;
   uint8_t* dbase = dst + (long long)J.tc * dpitch + (long long)J.lc * sizeof(T);
-  constexpr int NW = NT / 64, RH = BH / NW, NQ = (TShape<BW, BH, NT>::BPW * BW + 63) / 64;
+  constexpr int NW = NT / 64, RH = BH / NW, NQ = (TShape<BW, BH, 128>::BPW * BW + 63) / 64;
   constexpr int OC = row_chunk<RH>();  // rows per chunk
   const int l = t & 63, r0 = wave_of(t) * RH;  // lane in the wavefront; its first row (scalar)
   static_for<0, NQ>([&](auto pp) {
@@ -349,6 +349,55 @@ struct TabBuf {
   float wdh[SHP == SHP_GENERAL ? CH : 1];
 };
 
+// ------------------------------------------------------- split column FFT
+// The value of the other lane of a lane pair (lanes 2c, 2c+1): a DPP
+// quad permutation [1,0,3,2], one VALU instruction per 32-bit value.
+__device__ __forceinline__ float pair_swap(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float2 pair_swap(float2 v) { return make_float2(pair_swap(v.x), pair_swap(v.y)); }
+
+// Forward BH-point column DFT split over a lane pair: lane h holds the
+// BH/2-point DFT of the column's rows h, h+2, h+4, ... (E for h = 0, O for
+// h = 1).  X[k] = E[k] + w^k O[k] and X[k + BH/2] = E[k] - w^k O[k]
+// (w = exp(-2 pi i / BH)): lane 1 twiddles, the pair exchanges, lane 0 keeps
+// X[k] and lane 1 X[k + BH/2], k < BH/2.  `sg` is +1 on lane 0, -1 on lane 1
+// (fma by +-1 is an exact add / subtract).
+template <int BH>
+__device__ __forceinline__ void split_combine_fwd(float2 (&c)[BH / 2], int h, float sg) {
+  static_for<0, BH / 2>([&](auto kk) {
+    constexpr int k = decltype(kk)::value;
+    float2 v = c[k];
+    if constexpr (k != 0) {
+      constexpr float wr = (float)ct::cos2pi(k, BH), wi = (float)(-ct::sin2pi(k, BH));
+      const float2 tw = cmul(v, wr, wi);
+      v = h ? tw : v;
+    }
+    const float2 p = pair_swap(v);
+    c[k] = make_float2(__builtin_fmaf(sg, v.x, p.x), __builtin_fmaf(sg, v.y, p.y));
+  });
+}
+
+// Inverse of the above (unnormalised, FFTW c2r convention): lane 0 holds
+// X[k], lane 1 X[k + BH/2]; the even output rows are the BH/2-point inverse
+// DFT of X[k] + X[k + BH/2], the odd rows that of w^-k (X[k] - X[k + BH/2]).
+// Lane h ends with the inputs of its BH/2-point inverse transform.
+template <int BH>
+__device__ __forceinline__ void split_combine_inv(float2 (&c)[BH / 2], int h, float sg) {
+  static_for<0, BH / 2>([&](auto kk) {
+    constexpr int k = decltype(kk)::value;
+    const float2 v = c[k];
+    const float2 p = pair_swap(v);
+    float2 d = make_float2(__builtin_fmaf(sg, v.x, p.x), __builtin_fmaf(sg, v.y, p.y));
+    if constexpr (k != 0) {
+      constexpr float wr = (float)ct::cos2pi(k, BH), wi = (float)ct::sin2pi(k, BH);
+      const float2 tw = cmul(d, wr, wi);
+      d = h ? tw : d;
+    }
+    c[k] = d;
+  });
+}
+
 // ------------------------------------------------------------------ kernel
 // Register arrays (row and column transforms, chunk buffers) are computed by
 // every lane, idle lanes on a duplicate of a valid column (indices clamped to
@@ -359,23 +408,50 @@ struct TabBuf {
 // MF: the 2D transforms as DFT-matrix products on the matrix cores (32x32
 // blocks, nf3d_mfma.h), a wavefront per block in turn, in place of the row
 // and column FFTs; the spectral pass is unchanged.
+//
+// NT = 256 (FFT form): the tile of the 128-lane geometry (BPW blocks) with
+// twice the lanes.  Gather and overlap-add split the tile's rows over four
+// wavefronts; the row transforms use the first BPW * BH/2 lanes (the other
+// wavefronts skip them); each spectral column has a lane pair, which splits
+// its transform in two half-length transforms (split_combine_*) and runs the
+// spectral pass over half of the column's rows each.
+//
+// Waves per SIMD the register budget is cut for (256 lanes: LDS allows three
+// workgroups per CU at 48x48): three, or two for the variants whose spectral
+// pass holds more per element (four or five frames, the dehalo factor, the
+// Kalman state with a sharpen pass, 64-row columns), which would otherwise
+// spill.
+template <int BH, int NT, int MODE, int BT, int SHP>
+constexpr int run_blocks_per_cu() {
+  if constexpr (NT != 256) return 2;
+  else if constexpr (BH >= 64 || (MODE == MODE_W3D && (BT >= 4 || SHP == SHP_GENERAL)) ||
+                     (MODE == MODE_KALMAN && SHP != SHP_NONE))
+    return 2;
+  else return 3;
+}
+
 template <int BW, int BH, int NT, int MODE, int BT, int SHP, bool MF = false>
-__global__ void __launch_bounds__(NT, 2) k_run(RunBatch B_arg) {
+__global__ void __launch_bounds__(NT, (run_blocks_per_cu<BH, NT, MODE, BT, SHP>())) k_run(RunBatch B_arg) {
   const RunBatch& B0 = kernarg_batch(B_arg);
   const int jb = batch_job(B0, blockIdx.x);
   const RunLaunch& L0 = B0.job[jb];
-  using S = TShape<BW, BH, NT>;
+  using S = TShape<BW, BH, 128>;
+  constexpr bool SPLIT = NT == 256;
+  static_assert(NT == 128 || (NT == 256 && !MF), "256 lanes: the FFT form");
+  constexpr int CR = SPLIT ? BH / 2 : BH;  // spectral rows per column lane
   constexpr int COLS = S::COLS, PAIRS = S::PAIRS, BPW = S::BPW, SR = S::SR, LSPEC = S::LSPEC;
   constexpr int SYNW = S::SYNW, LSYN = S::LSYN, SA = S::SA, LA = S::LA;
   constexpr int FROM = MODE == MODE_W3D ? -(BT / 2) : 0;
   constexpr int TO = MODE == MODE_W3D ? (BT - 1) / 2 : 0;
   constexpr int NQ = MODE == MODE_W3D ? BT - 1 : 0;  // ring frames per element
   constexpr int CH = (MODE == MODE_W3D) ? 4 : 2;  // rows per chunk (two chunks of loads in flight)
-  constexpr int NCH = BH / CH;
-  static_assert(BH % CH == 0 && BH % 8 == 0 && BW % 4 == 0, "block shape");
+  constexpr int NCH = CR / CH;
+  static_assert(CR % CH == 0 && BH % 8 == 0 && BW % 4 == 0, "block shape");
   static_assert(MODE != MODE_FWD, "k_run filters");
   // chunks of ring rows in flight ahead of the one being computed
-  constexpr int DR = MODE == MODE_W3D ? (NQ <= 2 ? 3 : NQ == 3 ? 2 : 1) : 1;
+  // (256 lanes: half the rows per lane and a register budget of 168 for
+  // three wavefronts per SIMD)
+  constexpr int DR = MODE == MODE_W3D ? (SPLIT ? (NQ <= 2 ? 2 : 1) : (NQ <= 2 ? 3 : NQ == 3 ? 2 : 1)) : 1;
   using RBuf = RingBuf<MODE, NQ, CH>;
   using TBuf = TabBuf<SHP, CH>;
   __shared__ __align__(16) unsigned char smem[S::BYTES];
@@ -456,17 +532,28 @@ __global__ void __launch_bounds__(NT, 2) k_run(RunBatch B_arg) {
     const int* ro = rowoff + zo;
     const int m = L.m0 + fi;
     const int pix = J.pix;
-    const int cbr = t / COLS, cc = t - cbr * COLS;
+    // column lanes: one per column, or a pair (SPLIT: hh = half, rows
+    // hh * CR .. hh * CR + CR - 1 of the spectrum after the forward transform)
+    const int cl = SPLIT ? (t >> 1) : t;
+    const int hh = SPLIT ? (t & 1) : 0;
+    const float sg = hh ? -1.0f : 1.0f;
+    const int cbr = cl / COLS, cc = cl - cbr * COLS;
     const bool col_on = cbr < nblk;
     const int cb = col_on ? cbr : nblk - 1;
-    const int rbr = t / PAIRS, rp = t - rbr * PAIRS;
+    // row lanes: one per row pair, or a pair of lanes (SPLIT, half hh)
+    const int rl = SPLIT ? (t >> 1) : t;
+    const int rbr = rl / PAIRS, rp = rl - rbr * PAIRS;
     const bool row_on = rbr < nblk;
     const int rb = row_on ? rbr : nblk - 1;
+    // wavefronts with no row pair of the tile skip the row transforms
+    const bool row_wave = wave_of(t) * 64 < BPW * PAIRS * (SPLIT ? 2 : 1);
     const size_t tile_spec0 = (size_t)(by * J.nox + bx0) * (BH * COLS);
-    const size_t colo = tile_spec0 + (size_t)cb * (BH * COLS) + cc;  // element (0, cc) of this lane's column
+    // element (hh * CR, cc) of this lane's column: its first spectral row
+    const size_t colo = tile_spec0 + (size_t)cb * (BH * COLS) + (size_t)(hh * CR) * COLS + cc;
     // the pattern: one for the plane, or this block's (learned PSD)
     const float* pat = J.use_pattern ? (MODE == MODE_W3D ? J.pattern3d : J.pattern2d) +
-                                           (J.pattern_per_block ? colo - cc : 0)
+                                           (J.pattern_per_block ? colo - (size_t)(hh * CR) * COLS - cc : 0) +
+                                           (size_t)(hh * CR) * COLS
                                      : nullptr;
     const float grid0 = J.grid[0].x;
     const SpecParams SP(J);
@@ -503,6 +590,37 @@ __global__ void __launch_bounds__(NT, 2) k_run(RunBatch B_arg) {
         if (b >= 0)
           dft32_fwd_compute(din, sx, T, lane, [&](int r, int k, float2 zz) { spec[b * LSPEC + r * SR + k] = zz; });
       }
+    } else if (!row_wave) {
+      __syncthreads();  // gather stage dead; spectral layout from here
+    } else if constexpr (SPLIT) {
+      // ---- 2. row r2c, a lane pair per two real rows packed as one complex
+      // row Z: the BW-point transform as two BW/2-point transforms of its even
+      // and odd columns combined across the pair (lane 0: Z[k], lane 1:
+      // Z[k + BW/2]); after an exchange each lane finishes half of the output
+      // columns k (lane 0: 0..BW/4, lane 1: BW/4+1..BW/2) from Z[k] and
+      // Z[BW - k].
+      constexpr int HW = BW / 2, K0 = HW / 2 + 1;
+      float2 z[HW];
+      static_for<0, HW>([&](auto j) { z[j] = stage[rb * LA + rp * SA + 2 * decltype(j)::value + hh]; });
+      __syncthreads();  // gather stage dead; spectral layout from here
+      fft<HW, -1>(z);
+      split_combine_fwd<BW>(z, hh, sg);
+      float2 o[HW];
+      static_for<0, HW>([&](auto j) { o[j] = pair_swap(z[j]); });
+      // lane 0: low half Z[0..HW) = z, high half = o; lane 1 the other way round
+      float2* dst = spec + rb * LSPEC + 2 * rp * SR + hh * K0;
+      static_for<0, K0>([&](auto qq) {
+        constexpr int q = decltype(qq)::value;
+        constexpr int k1 = K0 + q;  // lane 1's column
+        const float2 a0 = z[q], b0 = q == 0 ? z[0] : o[(HW - q) % HW];
+        const float2 a1 = k1 < HW ? o[k1 < HW ? k1 : 0] : z[0];
+        const float2 b1 = z[k1 <= HW ? HW - k1 : 0];
+        const float2 a = hh ? a1 : a0, b = hh ? b1 : b0;
+        if (row_on && (hh == 0 || k1 <= HW)) {
+          dst[q] = make_float2((a.x + b.x) * 0.5f, (a.y - b.y) * 0.5f);
+          dst[SR + q] = make_float2((a.y + b.y) * 0.5f, (b.x - a.x) * 0.5f);
+        }
+      });
     } else {
       // ---- 2. row r2c, two real rows per lane
       float2 z[BW];
@@ -537,11 +655,12 @@ __global__ void __launch_bounds__(NT, 2) k_run(RunBatch B_arg) {
         if constexpr (MODE == MODE_KALMAN) b.ks[u] = KalState{J.kal_last[o], J.kal_covar[o], J.kal_cproc[o]};
       });
     };
+    const int tab0 = hh * CR * COLS + cc;  // table element of the lane's first row
     auto load_tab = [&](auto kc, TBuf& b) {
       constexpr int k = decltype(kc)::value;
       static_for<0, CH>([&](auto uu) {
         constexpr int u = decltype(uu)::value, i = k * CH + u;
-        const int kk = i * COLS + cc;
+        const int kk = i * COLS + tab0;
         b.g[u] = J.grid[kk];
         if constexpr (SHP != SHP_NONE) b.wsh[u] = J.wsharpen[kk];
         if constexpr (SHP == SHP_GENERAL) b.wdh[u] = J.wdehalo[kk];
@@ -565,20 +684,27 @@ __global__ void __launch_bounds__(NT, 2) k_run(RunBatch B_arg) {
     };
     RBuf rbuf[DR + 1];
     TBuf tbuf[2];
-    float2 col[BH];
+    float2 col[CR];
     static_for<0, (DR < NCH ? DR : NCH)>([&](auto kc) { load_ring(kc, rbuf[decltype(kc)::value]); });
     load_tab(std::integral_constant<int, 0>{}, tbuf[0]);
-    static_for<0, BH>([&](auto i) { col[i] = spec[cb * LSPEC + decltype(i)::value * SR + cc]; });
-    if constexpr (!MF) fft<BH, -1>(col);
+    if constexpr (SPLIT) {
+      // even rows on lane 0, odd rows on lane 1
+      static_for<0, CR>([&](auto j) { col[j] = spec[cb * LSPEC + (2 * decltype(j)::value + hh) * SR + cc]; });
+      fft<CR, -1>(col);
+      split_combine_fwd<BH>(col, hh, sg);
+    } else {
+      static_for<0, BH>([&](auto i) { col[i] = spec[cb * LSPEC + decltype(i)::value * SR + cc]; });
+      if constexpr (!MF) fft<BH, -1>(col);
+    }
     if constexpr (MODE == MODE_W3D) {
       if (col_on) {
         float2* out = L.ring + (size_t)((m + TO) % BT) * L.spec_elems + colo;
-        static_for<0, BH>([&](auto i) { out[decltype(i)::value * COLS] = col[i]; });
+        static_for<0, CR>([&](auto i) { out[decltype(i)::value * COLS] = col[i]; });
       }
     }
     // ---- 4a. per-block degrid fractions from the input DC and from the
     // filtered DC (code_impl_C.h:64); lane 0 of each block computes both
-    if (col_on && cc == 0) {
+    if (col_on && cc == 0 && hh == 0) {
       const Tab T0 = tab(tbuf[0], std::integral_constant<int, 0>{});
       float gf = 0.0f, dcf;
       if constexpr (MODE == MODE_W3D) {
@@ -658,9 +784,17 @@ __global__ void __launch_bounds__(NT, 2) k_run(RunBatch B_arg) {
       });
     }
     // ---- 5. column inverse FFT, back to LDS for the row c2r
-    if constexpr (!MF) fft<BH, +1>(col);
-    if (col_on) {
-      static_for<0, BH>([&](auto i) { spec[cb * LSPEC + decltype(i)::value * SR + cc] = col[i]; });
+    if constexpr (SPLIT) {
+      split_combine_inv<BH>(col, hh, sg);
+      fft<CR, +1>(col);
+      if (col_on) {
+        static_for<0, CR>([&](auto j) { spec[cb * LSPEC + (2 * decltype(j)::value + hh) * SR + cc] = col[j]; });
+      }
+    } else {
+      if constexpr (!MF) fft<BH, +1>(col);
+      if (col_on) {
+        static_for<0, BH>([&](auto i) { spec[cb * LSPEC + decltype(i)::value * SR + cc] = col[i]; });
+      }
     }
     __syncthreads();
     if constexpr (MF) {
@@ -688,6 +822,54 @@ __global__ void __launch_bounds__(NT, 2) k_run(RunBatch B_arg) {
             d[2] = x[2];
             d[3] = x[3];
           });
+      }
+    } else if (!row_wave) {
+      __syncthreads();  // spectral layout dead; synthesised rows next
+    } else if constexpr (SPLIT) {
+      // ---- 6. row c2r, a lane pair per two rows: each lane builds the packed
+      // spectrum z (Hermitian rows packed, DC and Nyquist imaginary parts
+      // dropped) and reduces it to its half-length inverse transform's input
+      // (split_combine_inv: lane 0 z[k] + z[k + BW/2], lane 1
+      // w^-k (z[k] - z[k + BW/2])); lane h ends with output columns 2j + h.
+      constexpr int HW = BW / 2;
+      const float2* src = spec + rb * LSPEC + 2 * rp * SR;
+      auto zcol = [&](auto kk) {  // z[k] (k <= HW) from column k
+        constexpr int k = decltype(kk)::value;
+        float2 a = src[k], b = src[SR + k];
+        if constexpr (k == 0 || k == HW) {
+          a.y = 0.0f;
+          b.y = 0.0f;
+        }
+        return make_float2(a.x - b.y, a.y + b.x);
+      };
+      auto zmirror = [&](auto kk) {  // z[BW - k] (0 < k < HW) from column k
+        constexpr int k = decltype(kk)::value;
+        const float2 a = src[k], b = src[SR + k];
+        return make_float2(a.x + b.y, b.x - a.y);
+      };
+      float2 z[HW];
+      static_for<0, HW>([&](auto kk) {
+        constexpr int k = decltype(kk)::value;
+        const float2 zl = zcol(kk);
+        float2 zu;
+        if constexpr (k == 0) zu = zcol(std::integral_constant<int, HW>{});
+        else zu = zmirror(std::integral_constant<int, HW - k>{});
+        float2 d = make_float2(__builtin_fmaf(sg, zu.x, zl.x), __builtin_fmaf(sg, zu.y, zl.y));
+        if constexpr (k != 0) {
+          constexpr float wr = (float)ct::cos2pi(k, BW), wi = (float)ct::sin2pi(k, BW);
+          const float2 tw = cmul(d, wr, wi);
+          d = hh ? tw : d;
+        }
+        z[k] = d;
+      });
+      __syncthreads();  // spectral layout dead; synthesised rows next
+      fft<HW, +1>(z);
+      if (row_on) {
+        float* d0 = syn + rb * LSYN + 2 * rp * SYNW + hh;
+        static_for<0, HW>([&](auto j) {
+          d0[2 * decltype(j)::value] = z[j].x;
+          d0[SYNW + 2 * decltype(j)::value] = z[j].y;
+        });
       }
     } else {
       // ---- 6. row c2r (Hermitian rows packed in pairs; DC and Nyquist imag dropped)
@@ -813,7 +995,7 @@ __global__ void __launch_bounds__(256) k_seam_run(RunBatch B_arg) {
 
 template <int BW, int BH, bool MF>
 hipError_t launch_run_modes(int mode, int bt, int shp, const RunBatch& L, hipStream_t s) {
-  constexpr int NT = 128;
+  constexpr int NT = MF ? 128 : 256;
   dim3 grid(L.total_wg), block(NT);
 #define NF3D_RUN(M, B, H) hipLaunchKernelGGL((k_run<BW, BH, NT, M, B, H, MF>), grid, block, 0, s, L)
 #define NF3D_RUN_SHP(M, B)                                   \

@@ -29,8 +29,13 @@ def test_gray8_mfma_modes(product, oracle, args):
     check_clip(product, oracle, "gray", 8, False, args + ";dft=1", W=160, H=128)
 
 
-def test_rgb_float_mfma(product, oracle):
-    check_clip(product, oracle, "rgb", 32, True, "sigma=3;bt=3;bw=32;bh=32;dft=1")
+def test_rgb_float_mfma_refused(product):
+    """Float samples have no bounded range to scale the FP16 operands from:
+    the matrix-core form refuses them (the FFT form handles any value)."""
+    from harness import FilterError
+    clip = make_clip("rgb", 32, True, width=96, height=64, nframes=3, seed=3)
+    with pytest.raises(FilterError, match="integer formats only"):
+        product.open("sigma=3;bt=3;bw=32;bh=32;dft=1", clip)
 
 
 def test_yuv_p16_mfma(product, oracle):

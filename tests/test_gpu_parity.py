@@ -96,12 +96,6 @@ def test_yuv_p16_kalman(product, oracle):
                exact=0.99)
 
 
-def test_gray8_640x360_bt3_48(product, oracle):
-    """The CPU-runnable workload at full size: Gray8 640x360, bt=3, 48/16."""
-    check_clip(product, oracle, "gray", 8, False, "sigma=2;bt=3;bw=48;bh=48;ow=16;oh=16;sharpen=0",
-               W=640, H=360, F=30, frames=[0, 1, 2, 15, 28, 29])
-
-
 # ------------------------------------------------------------ golden vectors
 from test_oracle import GOLDEN, golden_bar, load_golden  # noqa: E402
 
