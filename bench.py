@@ -12,9 +12,8 @@ frame-loop launches through nf3d_run_device_batch).  Prints one JSON line:
   value         frames/s over the timed steps (whole job, one GPU)
   roofline      HIP-event time of the main spectral kernel against the HBM
                 peak, with the algorithmic bytes of that kernel
-  cpu_baseline  the reference's own C++ path (compiled from its sources into
-                oracle/_ref, AVX-512 kernels, opt=4) on a bounded sample of the
-                same workload on this host's cores
+  cpu_baseline  the FP64 C restatement of the path (oracle/) on a bounded
+                sample of the same workload on this host's cores
   psnr_db       denoised vs clean (noise-free render of the same seed)
   precision_form  the DFT-as-GEMM form (FP16 split operands on the matrix
                 cores) on 3840x2160 32/16 blocks: rate and error vs FP64
@@ -33,7 +32,6 @@ import argparse
 import ctypes
 import json
 import os
-import subprocess
 import sys
 import time
 
@@ -169,6 +167,24 @@ def gpu_run(args):
     achieved = bytes_total / (ms_total * 1e-3) / 1e9 if ms_total > 0 else 0.0
     avg_us = ms_total * 1e3 / max(frames_timed, 1)
 
+    # a plain device copy (1 GiB read + 1 GiB written) on the same box: the
+    # HBM rate this box reaches, and the fraction of it the filter reaches
+    cp = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
+    cq = torch.empty_like(cp)
+    cq.copy_(cp)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        cq.copy_(cp)
+    e1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 2 * 5 * (1 << 30) / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del cp, cq
+
+    # host-fed: the same frames from (pageable) host memory through the
+    # pinned, chunked upload / filter / download pipeline (PCIe-inclusive)
+    host = host_fed(planes, args)
+
     # quality: denoised vs a noise-free render of the same clip
     idx = [F // 2 + k for k in range(4)]
     clean = rects_clip(W, H, F, SEED_BASE + 1, noise_luma=0.0, noise_chroma=0.0, threads=16)
@@ -193,11 +209,42 @@ def gpu_run(args):
             "traffic_over_algorithmic": traffic_ratio,
             "traffic_source": TRAFFIC_PROFILE + " (FETCH_SIZE x2.00, WRITE_SIZE x1.00 by a 1 GiB copy)",
             "kernel": f"k_run<48,48,MODE_W3D,bt={args.bt},sharpen> + k_seam_run (3 planes per launch)",
+            "copy_gbs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4),
             "us_per_frame_plane": round(avg_us, 2), "frame_planes": frames_timed,
             "bytes_model": "per frame-plane: 2*W*H*Bps + bt*nb*bh*(bw/2+1)*8",
         },
         "psnr_db": q,
+        "host_fed": host,
     }
+
+
+def host_fed(planes, args):
+    """Frames/s of the whole clip handed over in host memory (numpy planes)
+    and returned to host memory by nf3d_run_host_batch: the caller's planes
+    copied to pinned staging by host threads, uploaded, filtered and
+    downloaded chunk by chunk on three streams."""
+    import numpy as np
+
+    from neo_fft3d_amd import Engine, default_params, make_format, run_host_batch
+    F = args.frames
+    fmt = make_format("yuv", 8)
+    prm = default_params(**FILTER, bt=args.bt)
+    engines = [Engine(prm, fmt, W, H, F, p) for p in range(3)]
+    outs = [np.empty_like(x) for x in planes]
+    items = [(engines[p], 0, F, [planes[p][n].ctypes.data for n in range(F)], planes[p].strides[1],
+              [outs[p][n].ctypes.data for n in range(F)], outs[p].strides[1]) for p in range(3)]
+    run_host_batch(items)  # warm-up (allocations, first transforms)
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        run_host_batch(items)
+    dt = time.perf_counter() - t0
+    for e in engines:
+        e.close()
+    mb = sum(x[0].nbytes for x in planes) / 1e6
+    return {"fps": round(F * reps / dt, 1), "unit": "frames/s",
+            "note": f"PCIe-inclusive: {mb:.2f} MB in and out per frame from pageable host memory; "
+                    f"{F} frames x {reps} passes"}
 
 
 def precision_form(args):
@@ -488,7 +535,9 @@ def learned_form(args):
     torch.cuda.synchronize()
     ms = a.elapsed_time(b) / 20
     flops = 2 * nb * (mlp.n_in * mlp.nh + mlp.nh * mlp.nh + mlp.nh * mlp.n_out)
+    metric_cfg = learned_on_metric_config(args)
     return {
+        "metric_config": metric_cfg,
         "workload": "rgbps_float_bt3_bw48_ow16, coloured noise (power law x anisotropy, Perlin sigma 0.005-0.04)",
         "train": {"blocks": data.rows, "steps": 1500, "batch": 4096, "seconds": round(t_train, 1),
                   "whittle_loss": [round(losses[0][1], 4), round(losses[-1][1], 4)]},
@@ -502,65 +551,117 @@ def learned_form(args):
     }
 
 
+def learned_on_metric_config(args):
+    """The learned-PSD form on the headline workload itself (1080p YUV420P8,
+    300 frames, bt=3, 48/16, sharpen 0.3, all planes): a network for 8-bit
+    48x48 blocks trained on the GPU (seeded coloured-noise data in 8-bit
+    units), then frames/s of plain and learned over the resident clip in one
+    batched call each, and the luma PSNR of both against the clean render."""
+    import numpy as np
+    import torch
+
+    from neo_fft3d_amd import Engine, default_params, make_format, run_device_batch
+    from neo_fft3d_amd import learned as L
+    from neo_fft3d_amd.synth import SEED_BASE, rects_clip
+    t0 = time.time()
+    mlp, data, losses = L.train_model(clips=24, frames=2, width=384, height=384, steps=800, scale=255.0)
+    t_train = time.time() - t0
+    F = args.frames
+    planes = rects_clip(W, H, F, SEED_BASE + 1)
+    clean = rects_clip(W, H, F, SEED_BASE + 1, noise_luma=0.0, noise_chroma=0.0)
+    src = [torch.from_numpy(p).cuda() for p in planes]
+    dst = [torch.empty_like(x) for x in src]
+    fmt = make_format("yuv", 8)
+
+    def run(learned, strength=1.0, reps=3):
+        prm = default_params(**FILTER, bt=3)
+        es = [Engine(prm, fmt, W, H, F, p) for p in range(3)]
+        if learned:
+            for e in es:
+                L.set_learned(e, mlp, strength)
+        items = [(es[p], 0, F, [src[p][n].data_ptr() for n in range(F)], src[p].stride(1),
+                  [dst[p][n].data_ptr() for n in range(F)], dst[p].stride(1)) for p in range(3)]
+        torch.cuda.synchronize()
+        run_device_batch(items)  # warm-up
+        for e in es:
+            e.sync()
+        t = time.perf_counter()
+        for _ in range(reps):
+            run_device_batch(items)
+        for e in es:
+            e.sync()
+        torch.cuda.synchronize()
+        fps = reps * F / (time.perf_counter() - t)
+        idx = [F // 2 + k for k in range(4)]
+        out = dst[0].cpu().numpy()
+        db = float(np.mean([psnr(out[n], clean[0][n]) for n in idx]))
+        for e in es:
+            e.close()
+        return fps, db
+    fps_plain, db_plain = run(False)
+    fps_learned, db_learned = max((run(True, a) for a in (1.0, 1.5)), key=lambda r: r[1])
+    return {"workload": "yuv420p8_1920x1080_300f_bt3_bw48_ow16_sharpen0.3",
+            "train": {"blocks": data.rows, "steps": 800, "seconds": round(t_train, 1), "scale": 255.0},
+            "fps_plain": round(fps_plain, 1), "fps_learned": round(fps_learned, 1),
+            "learned_over_plain": round(fps_learned / fps_plain, 3),
+            "psnr_luma_plain": round(db_plain, 2), "psnr_luma_learned": round(db_learned, 2)}
+
+
 # ------------------------------------------------------------- CPU baseline
-def cpu_worker(args):
-    """Runs in a child process (MKL threading layer set before it loads)."""
+def cpu_baseline(args):
+    """The FP64 restatement of the path (oracle/libnf3d_oracle.so, scalar C)
+    on a bounded sample of the same workload: the first `threads * per` frames
+    of the seeded clip, each thread with its own oracle instance filtering a
+    contiguous run of `per` frames (its window's neighbours read from the
+    shared input).  The reference's own build needs FFTW, which this host
+    lacks, so the restatement is the CPU figure."""
+    import threading
+
     import numpy as np
 
     from neo_fft3d_amd.synth import SEED_BASE, rects_clip
 
     threads = args.cpu_threads
-    F = args.cpu_frames
+    per = args.cpu_frames_per_thread
+    F = threads * per
     planes = rects_clip(W, H, F, SEED_BASE + 1, threads=threads)
-    lib = ctypes.CDLL(os.path.join(ROOT, "oracle", "_ref", "libnf3d_ref.so"))
-    lib.nf3dref_create.restype = ctypes.c_void_p
-    lib.nf3dref_create.argtypes = [ctypes.c_char_p] + [ctypes.c_int] * 8 + [ctypes.c_void_p, ctypes.c_void_p,
-                                                                               ctypes.c_char_p, ctypes.c_int]
-    lib.nf3dref_run_parallel.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                         ctypes.c_char_p, ctypes.c_int]
+    lib = ctypes.CDLL(os.path.join(ROOT, "oracle", "libnf3d_oracle.so"))
+    lib.nf3d_oracle_create.restype = ctypes.c_void_p
+    lib.nf3d_oracle_create.argtypes = [ctypes.c_char_p] + [ctypes.c_int] * 8 + [ctypes.c_void_p, ctypes.c_void_p,
+                                                                                 ctypes.c_char_p, ctypes.c_int]
+    lib.nf3d_oracle_get_frame.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_char_p, ctypes.c_int]
+    lib.nf3d_oracle_free.argtypes = [ctypes.c_void_p]
     ptrs = (ctypes.c_void_p * (3 * F))(*[planes[p][n].ctypes.data for n in range(F) for p in range(3)])
     strides = (ctypes.c_int * 3)(*[planes[p].strides[1] for p in range(3)])
-    err = ctypes.create_string_buffer(256)
-    if args.cpu_variant == "parallel":
-        a = f"{REF_ARGS};bt={args.bt};opt=4;ncpu=1;mt=0"
-        host_threads = threads
-    else:
-        a = f"{REF_ARGS};bt={args.bt};opt=4;ncpu={threads};mt=1"
-        host_threads = 1
-    h = lib.nf3dref_create(a.encode(), 1, 0, 8, 1, 1, W, H, F, ptrs, strides, err, 256)
-    if not h:
-        raise RuntimeError(err.value.decode())
-    # warm-up (plans, caches), then time whole passes until the budget is used
-    lib.nf3dref_run_parallel(ctypes.c_void_p(h), 0, min(F, 2 * host_threads), host_threads, err, 256)
-    done = 0
+    a = f"{REF_ARGS};bt={args.bt}".encode()
+    errors = []
+
+    def work(t):
+        err = ctypes.create_string_buffer(256)
+        h = lib.nf3d_oracle_create(a, 1, 0, 8, 1, 1, W, H, F, ptrs, strides, err, 256)
+        if not h:
+            errors.append(err.value.decode())
+            return
+        outs = [np.zeros_like(planes[p][0]) for p in range(3)]
+        optr = (ctypes.c_void_p * 3)(*[o.ctypes.data for o in outs])
+        ostr = (ctypes.c_int * 3)(*[o.strides[0] for o in outs])
+        for n in range(t * per, (t + 1) * per):
+            if lib.nf3d_oracle_get_frame(ctypes.c_void_p(h), n, optr, ostr, err, 256) != 0:
+                errors.append(err.value.decode())
+                break
+        lib.nf3d_oracle_free(ctypes.c_void_p(h))
+
     t0 = time.perf_counter()
-    while time.perf_counter() - t0 < args.cpu_seconds:
-        if lib.nf3dref_run_parallel(ctypes.c_void_p(h), 0, F, host_threads, err, 256) != 0:
-            raise RuntimeError(err.value.decode())
-        done += F
+    pool = [threading.Thread(target=work, args=(t,)) for t in range(threads)]
+    for th in pool:
+        th.start()
+    for th in pool:
+        th.join()
     el = time.perf_counter() - t0
-    print(json.dumps({"fps": done / el, "frames": done, "seconds": el, "threads": threads,
-                      "variant": args.cpu_variant}))
-
-
-def cpu_baseline(args):
-    threads = args.cpu_threads
-    res = []
-    for variant, layer in (("parallel", "SEQUENTIAL"), ("sequential", "GNU")):
-        env = dict(os.environ, MKL_THREADING_LAYER=layer, MKL_NUM_THREADS=str(threads))
-        cmd = [sys.executable, os.path.abspath(__file__), "--cpu-worker", "--cpu-variant", variant,
-               "--cpu-threads", str(threads), "--cpu-frames", str(args.cpu_frames),
-               "--cpu-seconds", str(args.cpu_seconds / 2), "--bt", str(args.bt)]
-        r = subprocess.run(cmd, env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=600)
-        if r.returncode != 0:
-            res.append({"variant": variant, "error": r.stderr.strip()[-300:]})
-            continue
-        res.append(json.loads(r.stdout.strip().splitlines()[-1]))
-    ok = [x for x in res if "fps" in x]
-    if not ok:
-        return {"value": None, "unit": "frames/s", "cores": threads, "kind": "reference",
-                "sample": "failed: " + "; ".join(x.get("error", "") for x in res)}
-    best = max(ok, key=lambda x: x["fps"])
+    if errors:
+        return {"value": None, "unit": "frames/s", "cores": threads, "kind": "port",
+                "sample": "failed: " + errors[0]}
     cpu = ""
     try:
         for line in open("/proc/cpuinfo"):
@@ -569,12 +670,10 @@ def cpu_baseline(args):
                 break
     except OSError:
         pass
-    desc = (f"reference C++ path (oracle/_ref, opt=4 AVX-512, MKL FFTW3 interface) on "
-            f"{best['frames']} frames ({args.cpu_frames}-frame cycle) of the same 1080p YUV420P8 bt={args.bt} "
-            f"workload, {best['variant']} variant, {threads} threads on {cpu}; "
-            + ", ".join(f"{x['variant']}={x['fps']:.1f} fps" for x in ok))
-    return {"value": round(best["fps"], 2), "unit": "frames/s", "cores": threads, "kind": "reference",
-            "sample": desc}
+    desc = (f"FP64 scalar C restatement of the path (oracle/libnf3d_oracle.so) on the first {F} frames of "
+            f"the same 1080p YUV420P8 bt={args.bt} 48/16 sharpen 0.3 workload, all three planes: {threads} "
+            f"threads x {per} consecutive frames each, {el:.1f} s on {cpu}")
+    return {"value": round(F / el, 3), "unit": "frames/s", "cores": threads, "kind": "port", "sample": desc}
 
 
 def main():
@@ -594,16 +693,10 @@ def main():
     ap.add_argument("--multi-frames", type=int, default=24)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run one device copy of MIB MiB first (a reference kernel for traffic counters)")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--cpu-frames", type=int, default=32)
+    ap.add_argument("--cpu-frames-per-thread", type=int, default=4)
     ap.add_argument("--cpu-threads", type=int, default=min(os.cpu_count() or 1, 16))
-    ap.add_argument("--cpu-worker", action="store_true")
-    ap.add_argument("--cpu-variant", default="parallel")
     args = ap.parse_args()
 
-    if args.cpu_worker:
-        cpu_worker(args)
-        return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is a design note, not a code path)"}))

@@ -56,7 +56,10 @@ typedef struct nf3d_format {
   int is_float;        /* 32-bit float samples */
   int bits_per_sample; /* 8, 9..16 (uint16 storage), 32 */
   int ssw, ssh;        /* chroma subsampling shifts (yuv) */
-  int num_planes;      /* 1 or 3 */
+  int num_planes;      /* 1 (gray), 3, or 4: YUVA / RGBA, the alpha plane at
+                        * the luma size; copied unless the `planes` list
+                        * names it (the AviSynth+ y/u/v modes never do,
+                        * src/fft3d.hpp:149-169) */
 } nf3d_format;
 
 /* One plane of one frame.  `data` is host memory for the *_get_frame calls
@@ -119,6 +122,19 @@ typedef struct {
   int dst_stride;
 } nf3d_run_item;
 int nf3d_run_device_batch(const nf3d_run_item* items, int n, void* stream, char* err, size_t errlen);
+
+/* nf3d_run_device_batch with the frames in host memory (no reference
+ * counterpart: the host-fed form of FFT3D::GetFrame driven over a range,
+ * src/fft3d.hpp:201-264).  Every item covers the same frame range
+ * [first, first+count) of clips of the same length; src[k] / dst[k] are host
+ * planes of frame k (the source for every frame the filter windows read,
+ * up to two beyond each end of the range).  The frames stream through the
+ * device in chunks: the caller's planes are copied into pinned staging by
+ * host threads and uploaded on one stream while the previous chunk is
+ * filtered on another and the one before is downloaded on a third.  Only the
+ * processed region (the plane minus the l/t/r/b crop) of dst is written.
+ * Returns when every frame is in dst. */
+int nf3d_run_host_batch(const nf3d_run_item* items, int n, char* err, size_t errlen);
 
 /* The work of one nf3d_run_device_batch call captured as a HIP graph on
  * `stream` (a non-null hipStream_t) instead of being run; nf3d_graph_launch
@@ -197,6 +213,17 @@ int nf3d_engine_spectra(nf3d_engine* e, const void* src, int src_stride, void* s
  * frames (frames per launch: nf3d_engine_set_tuning; 1 gives the per-frame
  * semantics). */
 int nf3d_engine_set_learned(nf3d_engine* e, nf3d_mlp* m, float strength, char* err, size_t errlen);
+
+/* Spatially varying noise pattern (no reference counterpart; it generalises
+ * the pattern estimate of src/helper.cpp:46-132): with pfactor != 0 the
+ * block grid is cut into regions of rx x ry blocks, each region takes the
+ * pattern of its own first strict minimum of FindPatternBlock's sums over
+ * the search range's blocks inside it (or the plane's choice when it has
+ * none), and every block is filtered with its region's pattern (times bt in
+ * the 3D gains).  rx = ry = 0 restores the single pattern.  Takes effect at
+ * the next estimate (the next call, on frame pframe).  Errors: no estimated
+ * pattern (pfactor == 0, pattern display, sigma2..sigma4), sizes. */
+int nf3d_engine_set_pattern_regions(nf3d_engine* e, int rx, int ry, char* err, size_t errlen);
 
 /* Precision form of the engine's 2D transforms (no reference counterpart; the
  * reference uses FFTW): mode 0 = FP32 FFT in registers (default), 1 = dense
@@ -288,8 +315,9 @@ void nf3d_filter_destroy(nf3d_filter* f);
 
 /* ------------------------------------------------------------ clip-level
  * A whole in-memory clip behind the front end, with the same signature as the
- * oracle and reference test drivers (family 0 gray / 1 yuv / 2 rgb,
- * sample_type 0 integer / 1 float, planes[frame * nplanes + plane]). */
+ * oracle and reference test drivers (family 0 gray / 1 yuv / 2 rgb, 3 yuva /
+ * 4 rgba with a fourth plane, sample_type 0 integer / 1 float,
+ * planes[frame * nplanes + plane]). */
 void* nf3d_clip_create(const char* args, int family, int sample_type, int bits, int ssw, int ssh,
                        int width, int height, int nframes, const void* const* planes,
                        const int* strides, char* err, int errlen);

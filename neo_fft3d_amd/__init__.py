@@ -88,6 +88,7 @@ def lib() -> ctypes.CDLL:
         L.nf3d_engine_run_device.argtypes = [vp, ip, ip, ctypes.POINTER(vp), ip, ctypes.POINTER(vp), ip, vp,
                                              ctypes.c_char_p, sz]
         L.nf3d_run_device_batch.argtypes = [ctypes.POINTER(RunItem), ip, vp, ctypes.c_char_p, sz]
+        L.nf3d_run_host_batch.argtypes = [ctypes.POINTER(RunItem), ip, ctypes.c_char_p, sz]
         L.nf3d_run_device_batch_graph.argtypes = [ctypes.POINTER(RunItem), ip, vp, ctypes.POINTER(vp),
                                                   ctypes.c_char_p, sz]
         L.nf3d_graph_launch.argtypes = [vp, vp]
@@ -157,6 +158,16 @@ class Engine:
                                         ctypes.c_void_p(stream), err, 512) != 0:
             raise Nf3dError(err.value.decode())
 
+    def set_pattern_regions(self, rx: int, ry: int):
+        """Per-region noise patterns: regions of rx x ry blocks (0, 0: one
+        pattern for the plane, as the reference)."""
+        L = lib()
+        L.nf3d_engine_set_pattern_regions.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                                      ctypes.c_char_p, ctypes.c_size_t]
+        err = ctypes.create_string_buffer(256)
+        if L.nf3d_engine_set_pattern_regions(self.h, rx, ry, err, 256) != 0:
+            raise Nf3dError(err.value.decode())
+
     def set_dft(self, mode: int):
         """2D transform form: 0 = FFT in registers, 1 = DFT-matrix products on
         the matrix cores (32x32 blocks)."""
@@ -209,6 +220,15 @@ def run_device_batch(items, stream: int = 0):
     arr, keep = _run_items(items)
     err = ctypes.create_string_buffer(512)
     if lib().nf3d_run_device_batch(arr, len(items), ctypes.c_void_p(stream), err, 512) != 0:
+        raise Nf3dError(err.value.decode())
+
+
+def run_host_batch(items):
+    """nf3d_run_host_batch: the batch with host planes (pointers to host
+    memory), streamed through the device; returns when dst holds every frame."""
+    arr, keep = _run_items(items)
+    err = ctypes.create_string_buffer(512)
+    if lib().nf3d_run_host_batch(arr, len(items), err, 512) != 0:
         raise Nf3dError(err.value.decode())
 
 

@@ -26,8 +26,10 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/nf3d.h"
@@ -151,6 +153,7 @@ struct nf3d_engine {
   float2* d_ring = nullptr;
   int ring_slots = 0;
   std::vector<int> ring_frame;
+  std::vector<char> ring_by_run;  // the slot's spectrum came from the frame-loop kernel
   std::vector<uint64_t> ring_tick;
   uint64_t tick = 0;
   float2 *d_kal_last = nullptr, *d_kal_covar = nullptr, *d_kal_cproc = nullptr;
@@ -198,6 +201,11 @@ struct nf3d_engine {
   void* d_lx = nullptr;
   float* d_lpat = nullptr;
   void* d_lwork = nullptr;
+  // per-region noise patterns (regions of preg_x x preg_y blocks; 0 = the
+  // reference's single pattern): one pattern per block, 2D and 3D
+  int preg_x = 0, preg_y = 0;
+  float* d_rpat2d = nullptr;
+  float* d_rpat3d = nullptr;
 
   ~nf3d_engine() {
     hipFree(d_tab);
@@ -216,6 +224,8 @@ struct nf3d_engine {
     hipFree(d_lx);
     hipFree(d_lpat);
     hipFree(d_lwork);
+    hipFree(d_rpat2d);
+    hipFree(d_rpat3d);
     for (auto* p : d_stage) hipFree(p);
     hipFree(d_dst);
     for (auto ev : ev_pool) hipEventDestroy(ev);
@@ -477,6 +487,7 @@ struct nf3d_engine {
       ring_slots = ep.bt + 2;
       d_ring = dev_alloc<float2>(spec_elems * ring_slots);
       ring_frame.assign(ring_slots, -1);
+      ring_by_run.assign(ring_slots, 0);
       ring_tick.assign(ring_slots, 0);
     }
     if (ep.bt == 0) {
@@ -527,6 +538,11 @@ struct nf3d_engine {
     J.wdehalo = d_wdehalo;
     J.pattern2d = d_pattern2d;
     J.pattern3d = d_pattern3d;
+    if (preg_x > 0) {  // per-region patterns, one per block
+      J.pattern2d = d_rpat2d;
+      J.pattern3d = d_rpat3d;
+      J.pattern_per_block = 1;
+    }
     J.sigma2d = sigma2d;
     J.lowlimit = (ep.beta - 1) / ep.beta;
     J.degrid = ep.degrid;
@@ -658,6 +674,7 @@ struct nf3d_engine {
       if (best < 0 || ring_tick[i] < ring_tick[best]) best = i;
     }
     ring_frame[best] = k;
+    ring_by_run[best] = 0;
     ring_tick[best] = ++tick;
     return best;
   }
@@ -693,6 +710,13 @@ struct nf3d_engine {
     P.sums = d_psums;
     P.choice = d_choice;
     check(nf3d::launch_pattern(P, c.s), "pattern kernels");
+    if (preg_x > 0) {
+      // the 3D pattern is the 2D one times the first 3D btcur, which is bt
+      // (src/fft3d_engine.h:625-630)
+      check(nf3d::launch_pattern_regions(P, preg_x, preg_y, d_rpat2d, d_rpat3d, ep.bt >= 2 ? (float)ep.bt : 1.0f,
+                                         c.s),
+            "region pattern kernel");
+    }
     int choice[2];
     check(hipMemcpyAsync(choice, d_choice, sizeof(choice), hipMemcpyDeviceToHost, c.s), "pattern");
     check(hipMemcpyAsync(pattern2d.data(), d_pattern2d, pattern2d.size() * sizeof(float),
@@ -780,7 +804,12 @@ struct nf3d_engine {
     F.src_pitch = pitch;
     F.spec_out = d_lspec;
     launch(nf3d::MODE_FWD, 0, F, s);
-    if (nf3d_mlp_features(learned, d_lspec, nb, d_lx, s)) fail("learned PSD: feature kernel");
+    learned_psd_spec(d_lspec, mult, s);
+  }
+
+  // ... from spectra already computed (a frame in the Wiener-3D ring)
+  void learned_psd_spec(const float2* spec, float mult, hipStream_t s) {
+    if (nf3d_mlp_features(learned, spec, nb, d_lx, s)) fail("learned PSD: feature kernel");
     char msg[256] = {0};
     if (nf3d_mlp_predict(learned, d_lx, nb, d_lpat, 1, mult * learned_strength, d_lwork, s, msg, sizeof msg))
       fail(std::string("learned PSD: ") + msg);
@@ -994,16 +1023,36 @@ struct nf3d_engine {
     if (P.mode == nf3d::MODE_W3D) {
       if (ep.pfactor != 0 && !learned) ensure_pattern3d(ep.bt, s);
       // the window's older frames of the first step, into slot (frame % bt),
-      // unless the slot holds that frame already (a previous call's frames)
-      for (int k = P.a + P.from; k < P.a + P.to; k++) {
-        if (ring_frame[k % ep.bt] == k) continue;
-        PlaneJob F = base_job();
-        F.src = static_cast<const uint8_t*>(P.src[k]);
-        F.src_pitch = P.src_pitch;
-        F.spec_out = slot_ptr(k % ep.bt);
-        launch(nf3d::MODE_FWD, 0, F, s);
-        ring_frame[k % ep.bt] = k;
-        ring_tick[k % ep.bt] = ++tick;
+      // by the frame-loop kernel's forward transforms (so every spectrum the
+      // loop reads has its arithmetic, whatever the calls' split), unless
+      // the slot holds that frame from that kernel already (a previous call)
+      int k0 = P.a + P.from;
+      while (k0 < P.a + P.to && ring_frame[k0 % ep.bt] == k0 && ring_by_run[k0 % ep.bt]) k0++;
+      if (k0 < P.a + P.to) {
+        nf3d::RunBatch B{};
+        RunLaunch& R = B.job[0];
+        R.J = base_job();
+        R.J.src_pitch = P.src_pitch;
+        R.J.dst_pitch = P.dst_pitch;
+        for (int k = k0; k < P.a + P.to; k++) R.src[k - k0] = static_cast<const uint8_t*>(P.src[k]);
+        R.ring = d_ring;
+        R.spec_elems = (long long)spec_elems;
+        R.m0 = k0;
+        R.nframes = P.a + P.to - k0;
+        R.fpw = R.nframes;
+        R.ntiles = noy * ntx;
+        R.wg_begin = 0;
+        B.njobs = 1;
+        B.total_wg = R.ntiles;
+        B.max_hc = Hc;
+        B.max_frames = R.nframes;
+        check(nf3d::launch_run(nf3d::MODE_FWD, ep.bt, nf3d::SHP_NONE, dft_mode, ep.bw, ep.bh, B, s),
+              "frame-loop forward transforms");
+        for (int k = k0; k < P.a + P.to; k++) {
+          ring_frame[k % ep.bt] = k;
+          ring_by_run[k % ep.bt] = 1;
+          ring_tick[k % ep.bt] = ++tick;
+        }
       }
     }
   }
@@ -1016,6 +1065,7 @@ struct nf3d_engine {
       std::fill(ring_frame.begin(), ring_frame.end(), -1);
       for (int k = std::max(0, P.b + P.to - ep.bt + 1); k <= P.b + P.to; k++) {
         ring_frame[k % ep.bt] = k;
+        ring_by_run[k % ep.bt] = 1;
         ring_tick[k % ep.bt] = ++tick;
       }
     }
@@ -1163,9 +1213,17 @@ void run_batch(const nf3d_run_item* items, int n, hipStream_t s, bool timed, boo
           nf3d_engine* e = items[k].engine;
           RunLaunch& R = B.job[B.njobs++];
           R = plans[k].launches[l];
-          if (e->learned)
-            e->learned_psd_plane(static_cast<const uint8_t*>(plans[k].src[R.m0]), plans[k].src_pitch,
-                                 plans[k].mode == nf3d::MODE_W3D ? (float)e->ep.bt : 1.0f, s);
+          if (e->learned) {
+            // the PSD of the launch's first frame m0; for Wiener 3D with
+            // bt >= 3 its spectrum is in ring slot m0 % bt already (frame m0
+            // is an older frame of its own window), so the prediction starts
+            // from there instead of transforming the frame again
+            if (plans[k].mode == nf3d::MODE_W3D && e->ep.bt >= 3)
+              e->learned_psd_spec(e->slot_ptr(R.m0 % e->ep.bt), (float)e->ep.bt, s);
+            else
+              e->learned_psd_plane(static_cast<const uint8_t*>(plans[k].src[R.m0]), plans[k].src_pitch,
+                                   plans[k].mode == nf3d::MODE_W3D ? (float)e->ep.bt : 1.0f, s);
+          }
           R.wg_begin = B.total_wg;
           B.total_wg += R.ntiles * ((R.nframes + R.fpw - 1) / R.fpw);
           B.max_hc = std::max(B.max_hc, e->Hc);
@@ -1228,12 +1286,239 @@ int nf3d_engine_run_device(nf3d_engine* e, int first, int count, const void* con
 // are kept here (per engine) and given to the engines when a replay has
 // actually run, while after the capture itself the engines' records say the
 // ring holds nothing known.
+// --------------------------------------------------------- host-fed ranges
+// nf3d_run_host_batch: the batch of nf3d_run_device_batch with its frames in
+// host memory, streamed through the device in chunks of kChunk frames.  For
+// chunk c the host copies the caller's planes of chunk c+1 into pinned
+// staging (worker threads) and queues their upload on the upload stream,
+// while the compute stream filters chunk c from device frame slots and the
+// download stream returns chunk c-1, whose pinned output the host then copies
+// to the caller.  Device slots: input frame k in slot k % kInSlots (a chunk
+// plus the Wiener window's frames on either side, and the next chunk being
+// uploaded), output frame k in slot k % (2 * kChunk).
+namespace {
+
+constexpr int kChunk = 16;
+constexpr int kInSlots = 3 * kChunk;
+
+struct HostPlane {
+  nf3d_engine* e = nullptr;
+  size_t row = 0;      // bytes of one row
+  int rows = 0;
+  size_t pitch = 0;    // device and pinned slot pitch (256-byte aligned)
+  uint8_t* d_in = nullptr;
+  uint8_t* d_out = nullptr;
+  uint8_t* h_in[2] = {nullptr, nullptr};   // pinned, one chunk each
+  uint8_t* h_out[2] = {nullptr, nullptr};
+  std::vector<const void*> dsrc;
+  std::vector<void*> ddst;
+  ~HostPlane() {
+    hipFree(d_in);
+    hipFree(d_out);
+    for (int i = 0; i < 2; i++) {
+      hipHostFree(h_in[i]);
+      hipHostFree(h_out[i]);
+    }
+  }
+};
+
+// Rows of several planes between strided buffers, split over host threads.
+struct CopyJob {
+  uint8_t* dst;
+  size_t dpitch;
+  const uint8_t* src;
+  size_t spitch;
+  size_t row;
+  int rows;
+};
+void run_copies(const std::vector<CopyJob>& jobs) {
+  if (jobs.empty()) return;
+  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const size_t nt = std::min<size_t>(hw, jobs.size());
+  auto work = [&](size_t t) {
+    for (size_t j = t; j < jobs.size(); j += nt) {
+      const CopyJob& c = jobs[j];
+      if (c.dpitch == c.spitch && c.dpitch == c.row) {
+        memcpy(c.dst, c.src, c.row * c.rows);
+      } else {
+        for (int y = 0; y < c.rows; y++) memcpy(c.dst + y * c.dpitch, c.src + y * c.spitch, c.row);
+      }
+    }
+  };
+  std::vector<std::thread> th;
+  for (size_t t = 1; t < nt; t++) th.emplace_back(work, t);
+  work(0);
+  for (auto& t : th) t.join();
+}
+
+}  // namespace
+
 struct nf3d_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<nf3d_engine*> engines;
   std::vector<std::vector<int>> ring_after;  // ring_frame of each engine after one replay
+  std::vector<std::vector<char>> by_run_after;
 };
+
+int nf3d_run_host_batch(const nf3d_run_item* items, int n, char* err, size_t errlen) {
+  std::vector<hipStream_t> streams;
+  std::vector<hipEvent_t> events;
+  try {
+    auto locks = lock_batch(items, n);
+    int first = items[0].first, count = items[0].count;
+    for (int i = 0; i < n; i++) {
+      const nf3d_run_item& it = items[i];
+      if (it.first != first || it.count != count) fail("host batch items must share one frame range");
+      if (!it.src || !it.dst) fail("null frame list");
+      for (int k = std::max(0, first - 2); k < std::min(it.engine->frames, first + count + 2); k++)
+        if (!it.src[k] || (k >= first && k < first + count && !it.dst[k]))
+          fail("frame " + std::to_string(k) + " was not provided");
+    }
+    if (count < 1) return 0;
+    const int F = items[0].engine->frames;
+    for (int i = 0; i < n; i++)
+      if (items[i].engine->frames != F) fail("host batch engines must have the same clip length");
+    for (int k = 0; k < 3; k++) {
+      hipStream_t st;
+      check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+      streams.push_back(st);
+    }
+    hipStream_t s_up = streams[0], s_run = streams[1], s_down = streams[2];
+    // per chunk: uploaded, filtered, downloaded
+    const int nchunks = (count + kChunk - 1) / kChunk;
+    auto new_event = [&]() {
+      hipEvent_t ev;
+      check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
+      events.push_back(ev);
+      return ev;
+    };
+    std::vector<hipEvent_t> ev_up(nchunks + 1), ev_run(nchunks), ev_down(nchunks);
+    for (auto& e : ev_up) e = new_event();
+    for (auto& e : ev_run) e = new_event();
+    for (auto& e : ev_down) e = new_event();
+    std::vector<std::unique_ptr<HostPlane>> pl;
+    for (int i = 0; i < n; i++) {
+      nf3d_engine* e = items[i].engine;
+      auto P = std::make_unique<HostPlane>();
+      P->e = e;
+      P->row = (size_t)e->framewidth * e->pix;
+      P->rows = e->frameheight;
+      P->pitch = (P->row + 255) / 256 * 256;
+      const size_t slot = P->pitch * P->rows;
+      P->d_in = dev_alloc<uint8_t>(slot * kInSlots);
+      P->d_out = dev_alloc<uint8_t>(slot * 2 * kChunk);
+      for (int b = 0; b < 2; b++) {
+        check(hipHostMalloc((void**)&P->h_in[b], slot * (kChunk + 4), hipHostMallocDefault), "hipHostMalloc");
+        check(hipHostMalloc((void**)&P->h_out[b], slot * kChunk, hipHostMallocDefault), "hipHostMalloc");
+      }
+      P->dsrc.assign(F, nullptr);
+      P->ddst.assign(F, nullptr);
+      for (int k = 0; k < F; k++) {
+        P->dsrc[k] = P->d_in + (size_t)(k % kInSlots) * slot;
+        P->ddst[k] = P->d_out + (size_t)(k % (2 * kChunk)) * slot;
+      }
+      pl.push_back(std::move(P));
+    }
+    for (int i = 0; i < n; i++) items[i].engine->order_before(s_up);
+    // frames a chunk's filter reads: its own and the window's on either side
+    auto chunk_range = [&](int c, int& lo, int& hi) {
+      const int a = first + c * kChunk, b = std::min(first + count, a + kChunk) - 1;
+      lo = std::max(0, a - 2);
+      hi = std::min(F - 1, b + 2);
+    };
+    // frames chunk c uploads: those of its range not uploaded by chunk c-1
+    auto upload_range = [&](int c, int& lo, int& hi) {
+      chunk_range(c, lo, hi);
+      if (c > 0) {
+        int plo, phi;
+        chunk_range(c - 1, plo, phi);
+        lo = std::max(lo, phi + 1);
+      }
+    };
+    auto upload = [&](int c) {
+      int lo, hi;
+      upload_range(c, lo, hi);
+      const int buf = c & 1;
+      if (c >= 2) check(hipEventSynchronize(ev_up[c - 2]), "hipEventSynchronize");  // staging free again
+      std::vector<CopyJob> jobs;
+      for (int i = 0; i < n; i++) {
+        HostPlane& P = *pl[i];
+        for (int k = lo; k <= hi; k++)
+          jobs.push_back(CopyJob{P.h_in[buf] + (size_t)(k - lo) * P.pitch * P.rows, P.pitch,
+                                 static_cast<const uint8_t*>(items[i].src[k]), (size_t)items[i].src_stride, P.row,
+                                 P.rows});
+      }
+      run_copies(jobs);
+      // slots of frames the chunk two back still read are free once it ran
+      if (c >= 2) check(hipStreamWaitEvent(s_up, ev_run[c - 2], 0), "hipStreamWaitEvent");
+      for (int i = 0; i < n; i++) {
+        HostPlane& P = *pl[i];
+        for (int k = lo; k <= hi; k++)
+          check(hipMemcpyAsync(const_cast<void*>(P.dsrc[k]), P.h_in[buf] + (size_t)(k - lo) * P.pitch * P.rows,
+                               P.pitch * P.rows, hipMemcpyHostToDevice, s_up),
+                "upload");
+      }
+      check(hipEventRecord(ev_up[c], s_up), "hipEventRecord");
+    };
+    auto download_finish = [&](int c) {
+      check(hipEventSynchronize(ev_down[c]), "hipEventSynchronize");
+      const int a = first + c * kChunk, b = std::min(first + count, a + kChunk) - 1;
+      std::vector<CopyJob> jobs;
+      for (int i = 0; i < n; i++) {
+        HostPlane& P = *pl[i];
+        nf3d_engine* e = P.e;
+        // the processed region only: crop borders stay the caller's
+        for (int k = a; k <= b; k++)
+          jobs.push_back(CopyJob{static_cast<uint8_t*>(items[i].dst[k]) + (size_t)e->tc * items[i].dst_stride +
+                                     (size_t)e->lc * e->pix,
+                                 (size_t)items[i].dst_stride,
+                                 P.h_out[c & 1] + (size_t)(k - a) * P.pitch * P.rows + (size_t)e->tc * P.pitch +
+                                     (size_t)e->lc * e->pix,
+                                 P.pitch, (size_t)e->Wc * e->pix, e->Hc});
+      }
+      run_copies(jobs);
+    };
+    upload(0);
+    for (int c = 0; c < nchunks; c++) {
+      const int a = first + c * kChunk, cnt = std::min(first + count, a + kChunk) - a;
+      // filter chunk c once its frames are up
+      check(hipStreamWaitEvent(s_run, ev_up[c], 0), "hipStreamWaitEvent");
+      if (c >= 2) check(hipStreamWaitEvent(s_run, ev_down[c - 2], 0), "hipStreamWaitEvent");  // output slots
+      std::vector<nf3d_run_item> sub(n);
+      for (int i = 0; i < n; i++)
+        sub[i] = nf3d_run_item{items[i].engine, a, cnt, pl[i]->dsrc.data(), (int)pl[i]->pitch, pl[i]->ddst.data(),
+                               (int)pl[i]->pitch};
+      run_batch(sub.data(), n, s_run, true);
+      check(hipEventRecord(ev_run[c], s_run), "hipEventRecord");
+      // return it
+      check(hipStreamWaitEvent(s_down, ev_run[c], 0), "hipStreamWaitEvent");
+      for (int i = 0; i < n; i++) {
+        HostPlane& P = *pl[i];
+        for (int k = a; k < a + cnt; k++)
+          check(hipMemcpyAsync(P.h_out[c & 1] + (size_t)(k - a) * P.pitch * P.rows, P.ddst[k], P.pitch * P.rows,
+                               hipMemcpyDeviceToHost, s_down),
+                "download");
+      }
+      check(hipEventRecord(ev_down[c], s_down), "hipEventRecord");
+      // meanwhile: the next chunk's frames up, the previous chunk's out
+      if (c + 1 < nchunks) upload(c + 1);
+      if (c >= 1) download_finish(c - 1);
+    }
+    download_finish(nchunks - 1);
+    check(hipStreamSynchronize(s_up), "hipStreamSynchronize");
+    for (int i = 0; i < n; i++) items[i].engine->done_stream = nullptr;  // this call waited for its work
+  } catch (const Failure& f) {
+    for (auto st : streams) (void)hipStreamSynchronize(st);
+    for (auto ev : events) (void)hipEventDestroy(ev);
+    for (auto st : streams) (void)hipStreamDestroy(st);
+    copy_err(err, errlen, f.msg);
+    return -1;
+  }
+  for (auto ev : events) (void)hipEventDestroy(ev);
+  for (auto st : streams) (void)hipStreamDestroy(st);
+  return 0;
+}
 
 int nf3d_run_device_batch(const nf3d_run_item* items, int n, void* stream, char* err, size_t errlen) {
   try {
@@ -1299,6 +1584,7 @@ int nf3d_run_device_batch_graph(const nf3d_run_item* items, int n, void* stream,
       nf3d_engine* e = items[i].engine;
       g->engines.push_back(e);
       g->ring_after.push_back(e->ring_frame);
+      g->by_run_after.push_back(e->ring_by_run);
       std::fill(e->ring_frame.begin(), e->ring_frame.end(), -1);
     }
     hipError_t e1 = hipStreamEndCapture(s, &g->graph);
@@ -1339,7 +1625,10 @@ int nf3d_graph_launch(nf3d_graph* g, void* stream) {
     return -1;
   }
   // the replay leaves the frames recorded at capture in the rings
-  for (size_t i = 0; i < g->engines.size(); i++) g->engines[i]->ring_frame = g->ring_after[i];
+  for (size_t i = 0; i < g->engines.size(); i++) {
+    g->engines[i]->ring_frame = g->ring_after[i];
+    g->engines[i]->ring_by_run = g->by_run_after[i];
+  }
   return 0;
 }
 
@@ -1407,6 +1696,34 @@ int nf3d_engine_set_learned(nf3d_engine* e, nf3d_mlp* m, float strength, char* e
     if (e->run_frames_from_table) {
       e->run_frames = nf3d_engine::kRunFrames;
       e->run_frames_from_table = false;
+    }
+  } catch (const Failure& f) {
+    copy_err(err, errlen, f.msg);
+    return -1;
+  }
+  return 0;
+}
+
+int nf3d_engine_set_pattern_regions(nf3d_engine* e, int rx, int ry, char* err, size_t errlen) {
+  std::lock_guard<std::mutex> lock(e->mu);
+  try {
+    if (rx < 0 || ry < 0 || (rx == 0) != (ry == 0)) fail("pattern regions: both sizes >= 1 blocks, or both 0");
+    const bool sigmas = e->ep.sigma2 != e->ep.sigma || e->ep.sigma3 != e->ep.sigma || e->ep.sigma4 != e->ep.sigma;
+    if (rx > 0 && (e->ep.pfactor == 0 || e->ep.pshow))
+      fail("pattern regions need the estimated noise pattern (pfactor != 0, no pattern display)");
+    if (rx > 0 && sigmas && e->pattern_set && !e->d_psums)
+      fail("pattern regions need the estimated noise pattern, not one from sigma2..sigma4");
+    check(hipSetDevice(e->device), "hipSetDevice");
+    if (rx > 0 && !e->d_rpat2d) {
+      e->d_rpat2d = dev_alloc<float>(e->spec_elems);
+      e->d_rpat3d = dev_alloc<float>(e->spec_elems);
+    }
+    e->preg_x = rx;
+    e->preg_y = ry;
+    // estimate again (on the next call) with the new regions
+    if (e->d_psums) {
+      e->pattern_set = false;
+      e->pattern3d_init = false;
     }
   } catch (const Failure& f) {
     copy_err(err, errlen, f.msg);

@@ -233,6 +233,20 @@ int nf3d_filter_create(const char* args, const nf3d_format* fmt, int width, int 
     in.read("u", f->process[1]);
     in.read("v", f->process[2]);
   }
+  if (fmt->num_planes < 1 || fmt->num_planes > 4 || (fmt->color_family == 0 && fmt->num_planes != 1)) {
+    put_err(err, errlen, "formats have 1 (gray), 3 or 4 (with alpha) planes");
+    delete f;
+    return -1;
+  }
+  // the alpha plane of a YUV format is filtered with the chroma geometry
+  // (src/fft3d.hpp:186-191), which for a subsampled format covers part of
+  // the plane while the whole plane is copied out (:292-313): the reference
+  // leaves the rest undefined, so that case is refused
+  if (fmt->num_planes == 4 && f->process[3] == 3 && fmt->color_family == 1 && (fmt->ssw || fmt->ssh)) {
+    put_err(err, errlen, "the alpha plane of a subsampled YUVA clip cannot be filtered");
+    delete f;
+    return -1;
+  }
   for (int i = 0; i < fmt->num_planes; i++) {
     if (f->process[i] == 3) {
       std::string e(512, '\0');
@@ -250,6 +264,17 @@ int nf3d_filter_create(const char* args, const nf3d_format* fmt, int width, int 
         put_err(err, errlen, e.c_str());
         delete f;
         return -1;
+      }
+      // "pregion=rx[,ry]": per-region noise patterns (an extension)
+      std::vector<int> preg;
+      in.read_list("pregion", preg);
+      if (!preg.empty()) {
+        const int rx = preg[0], ry = preg.size() > 1 ? preg[1] : preg[0];
+        if (nf3d_engine_set_pattern_regions(f->eng[i], rx, ry, &e[0], e.size()) != 0) {
+          put_err(err, errlen, e.c_str());
+          delete f;
+          return -1;
+        }
       }
     } else if (f->process[i] == 2) {
       f->copy_count++;
@@ -336,12 +361,16 @@ void* nf3d_clip_create(const char* args, int family, int sample_type, int bits, 
                        int height, int nframes, const void* const* planes, const int* strides, char* err,
                        int errlen) {
   nf3d_format fmt{};
+  // families 3 / 4: YUVA / RGBA (a fourth, alpha plane at the luma size)
+  const int alpha = family == 3 || family == 4;
+  if (family == 3) family = 1;
+  if (family == 4) family = 2;
   fmt.color_family = family;
   fmt.is_float = sample_type != 0;
   fmt.bits_per_sample = bits;
   fmt.ssw = family == 1 ? ssw : 0;
   fmt.ssh = family == 1 ? ssh : 0;
-  fmt.num_planes = family == 0 ? 1 : 3;
+  fmt.num_planes = family == 0 ? 1 : 3 + alpha;
   auto* c = new nf3d_clip();
   c->nplanes = fmt.num_planes;
   c->frames = nframes;

@@ -134,6 +134,10 @@ hipError_t launch_tile(int mode, int bt, int onchip, int mf, int bw, int bh, con
 // Block shapes with a matrix-core DFT variant of k_tile (mf = 1).
 bool mfma_dft_supported(int bw, int bh);
 hipError_t launch_pattern(const PatternJob& job, hipStream_t s);
+// Per-region patterns after launch_pattern (which fills sums and choice):
+// every block gets its rx x ry region's pattern, 2D and times mult3d.
+hipError_t launch_pattern_regions(const PatternJob& job, int rx, int ry, float* out2d, float* out3d, float mult3d,
+                                  hipStream_t s);
 // DFT-as-GEMM precision form (nf3d_mfma_dft.hip): n x n blocks, n = 32.
 hipError_t launch_dft_mfma(const void* in, void* out, int nb, int n, int inverse, float scale, hipStream_t s);
 // k_run over the batch's jobs; `shp` is a Shp value.  launch_seam_run

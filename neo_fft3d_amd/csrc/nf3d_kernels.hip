@@ -59,6 +59,55 @@ __global__ void __launch_bounds__(256) k_pattern_set(PatternJob J) {
   }
 }
 
+// Spatially varying noise pattern (no reference counterpart; it generalises
+// FindPatternBlock / SetPattern, src/helper.cpp:46-132, to regions): the
+// block grid is cut into regions of rx x ry blocks, and each region takes the
+// pattern of its own first strict minimum of the same sums over the blocks
+// of the reference's search range (blocks 2..nox-3, 2..noy-3) that lie in
+// it -- or, for a region with none of them, the block chosen for the whole
+// plane (J.choice, written by k_pattern_set).  The pattern goes to every
+// block of the region: out2d = |X - g G|^2 pwin, out3d = out2d * mult3d.
+// One workgroup per region.
+__global__ void __launch_bounds__(256) k_pattern_regions(PatternJob J, int rx, int ry, float* out2d, float* out3d,
+                                                         float mult3d) {
+#pragma clang fp contract(off)
+  __shared__ int choice[2];
+  const int nqx = (J.nox + rx - 1) / rx;
+  const int qy = blockIdx.x / nqx, qx = blockIdx.x - qy * nqx;
+  const int x0 = qx * rx, x1 = min(J.nox, x0 + rx), y0 = qy * ry, y1 = min(J.noy, y0 + ry);
+  if (threadIdx.x == 0) {
+    int px = -1, py = -1;
+    float best = 1e15f;
+    for (int by = max(y0, 2); by < min(y1, J.noy - 2); by++)
+      for (int bx = max(x0, 2); bx < min(x1, J.nox - 2); bx++) {
+        const float v = J.sums[by * J.nox + bx];
+        if (v < best) { best = v; px = bx; py = by; }
+      }
+    if (px < 0) {
+      px = J.choice[0];
+      py = J.choice[1];
+    }
+    choice[0] = px;
+    choice[1] = py;
+  }
+  __syncthreads();
+  const int n = J.bh * J.outwidth;
+  const float2* blk = J.spec + ((size_t)choice[1] * J.nox + choice[0]) * n;
+  const float gcur = J.degrid * blk[0].x / J.grid[0].x;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const float2 g = J.grid[k];
+    const float c0 = blk[k].x - gcur * g.x, c1 = blk[k].y - gcur * g.y;
+    const float v = (c0 * c0 + c1 * c1) * J.pwin[k];
+    const float v3 = v * mult3d;
+    for (int by = y0; by < y1; by++)
+      for (int bx = x0; bx < x1; bx++) {
+        const size_t o = ((size_t)by * J.nox + bx) * n + k;
+        out2d[o] = v;
+        out3d[o] = v3;
+      }
+  }
+}
+
 // ---------------------------------------------------------------- dispatch
 // Block shapes compiled for k_tile (one translation unit each).
 #define NF3D_SHAPES(X) X(8, 8) X(16, 16) X(24, 24) X(32, 32) X(48, 48) X(64, 64) X(32, 16) X(16, 32)
@@ -112,6 +161,14 @@ hipError_t launch_pattern(const PatternJob& J, hipStream_t s) {
   int nb = J.nox * J.noy;
   hipLaunchKernelGGL(k_pattern_sums, dim3((nb + 127) / 128), dim3(128), 0, s, J);
   hipLaunchKernelGGL(k_pattern_set, dim3(1), dim3(256), 0, s, J);
+  return hipGetLastError();
+}
+
+hipError_t launch_pattern_regions(const PatternJob& J, int rx, int ry, float* out2d, float* out3d, float mult3d,
+                                  hipStream_t s) {
+  if (rx < 1 || ry < 1) return hipErrorInvalidValue;
+  const int nq = ((J.nox + rx - 1) / rx) * ((J.noy + ry - 1) / ry);
+  hipLaunchKernelGGL(k_pattern_regions, dim3(nq), dim3(256), 0, s, J, rx, ry, out2d, out3d, mult3d);
   return hipGetLastError();
 }
 

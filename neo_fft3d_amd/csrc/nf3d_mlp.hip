@@ -248,26 +248,34 @@ __global__ void __launch_bounds__(256) k_whittle(const float* Y, const float* T,
 }
 
 // Column sums of X [rows][cols]: pass 1 writes part[slice][col] over row
-// slices, pass 2 adds the slices.
-constexpr int CS_SLICES = 16;
+// slices of CS_ROWS rows (a workgroup: 64 columns x 4 row groups, each lane
+// with its CS_ROWS/4 loads in flight together), pass 2 adds the slices in
+// order (deterministic).  With 32-row slices a 4096-row batch gives 128
+// slices, so even 256 columns fill 512 workgroups.
+constexpr int CS_ROWS = 32;
 __global__ void __launch_bounds__(256) k_colsum1(const float* X, int rows, int cols, float* part) {
   __shared__ float red[4][64];
   const int c = blockIdx.x * 64 + (threadIdx.x & 63), q = threadIdx.x >> 6;
-  const int per = (rows + CS_SLICES - 1) / CS_SLICES;
-  const int r0 = blockIdx.y * per, r1 = min(rows, r0 + per);
+  const int r0 = blockIdx.y * CS_ROWS;
+  float v[CS_ROWS / 4];
+#pragma unroll
+  for (int u = 0; u < CS_ROWS / 4; u++) {
+    const int r = r0 + q + 4 * u;
+    v[u] = (c < cols && r < rows) ? X[(long long)r * cols + c] : 0.0f;
+  }
   float s = 0.0f;
-  if (c < cols)
-    for (int r = r0 + q; r < r1; r += 4) s += X[(long long)r * cols + c];
+#pragma unroll
+  for (int u = 0; u < CS_ROWS / 4; u++) s += v[u];
   red[q][threadIdx.x & 63] = s;
   __syncthreads();
   if (q == 0 && c < cols)
     part[blockIdx.y * cols + c] = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
 }
-__global__ void k_colsum2(const float* part, int cols, float* out) {
+__global__ void k_colsum2(const float* part, int cols, int slices, float* out) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= cols) return;
   float s = 0.0f;
-  for (int q = 0; q < CS_SLICES; q++) s += part[q * cols + c];
+  for (int q = 0; q < slices; q++) s += part[q * cols + c];
   out[c] = s;
 }
 
@@ -399,7 +407,7 @@ struct nf3d_mlp {
     for (void* q : {(void*)p, (void*)g, (void*)m, (void*)v, (void*)pb, (void*)mu, (void*)isd, (void*)z1, (void*)z2,
                     (void*)y, (void*)gf, (void*)part, (void*)lpart, (void*)h1, (void*)h2, (void*)d3, (void*)d2,
                     (void*)d1, (void*)wsk})
-      hipFree(q);
+      (void)hipFree(q);
   }
 
   void init(int ni, int h, int no, int mb) {
@@ -442,7 +450,7 @@ struct nf3d_mlp {
     z2 = dalloc<float>((size_t)mb * h);
     y = dalloc<float>((size_t)mb * no);
     gf = dalloc<float>((size_t)mb * wide);
-    part = dalloc<float>((size_t)CS_SLICES * wide);
+    part = dalloc<float>((size_t)((mb + CS_ROWS - 1) / CS_ROWS) * wide);
     lpart = dalloc<float>(blocks((long long)mb * no, 256));
     wsk = dalloc<float>((size_t)SPLIT_MAX * std::max({(size_t)ni * h, (size_t)h * h, (size_t)h * no}));
     h1 = dalloc<__bf16>((size_t)mb * h);
@@ -459,8 +467,9 @@ struct nf3d_mlp {
   }
 
   void colsum(const float* X, int rows, int cols, float* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_colsum1, dim3((cols + 63) / 64, CS_SLICES), dim3(256), 0, s, X, rows, cols, part);
-    hipLaunchKernelGGL(k_colsum2, dim3(blocks(cols, 256)), dim3(256), 0, s, part, cols, out);
+    const int slices = (rows + CS_ROWS - 1) / CS_ROWS;  // rows <= max_batch: part holds them
+    hipLaunchKernelGGL(k_colsum1, dim3((cols + 63) / 64, slices), dim3(256), 0, s, X, rows, cols, part);
+    hipLaunchKernelGGL(k_colsum2, dim3(blocks(cols, 256)), dim3(256), 0, s, part, cols, slices, out);
     mcheck(hipGetLastError(), "k_colsum");
   }
 

@@ -297,6 +297,118 @@ __device__ __forceinline__ void overlap_run(const PlaneJob& J, uint8_t* dst, flo
   });
 }
 
+// ------------------------------------------- gather / overlap-add by column
+// 256-lane tiles: lane (block b, column j) reads block b's column j of every
+// row straight from the plane (reflect-101 addressing) and writes it,
+// windowed, into the block's row pairs -- no per-pass column mapping, one
+// store per sample.  The window product is (p * wx) * wy for every sample:
+// the same value as the region-by-region order of gather_window up to one
+// rounding (the parity bar is against the FP64 product).
+template <int PIX, int BW, int BH, int SA, int LA>
+__device__ __forceinline__ void gather_cols(const PlaneJob& J, float2* stage, const uint8_t* src, const int* rowoff,
+                                            const float* winx, const float* winy, int bx0, int nblk, int stepx,
+                                            int t) {
+  using T = typename std::conditional<PIX == 1, uint8_t, typename std::conditional<PIX == 2, uint16_t, float>::type>::type;
+  const int b = t / BW, j = t - b * BW;
+  if (b >= nblk) return;  // lanes past the tile's blocks
+  const int sx = reflect101((bx0 + b) * stepx + j - J.mirw, J.Wc) + J.lc;
+  const float wx = winx[j], base = J.base;
+  T v[BH];
+  static_for<0, BH>([&](auto ii) {
+    constexpr int i = decltype(ii)::value;
+    v[i] = reinterpret_cast<const T*>(src + rowoff[i])[sx];
+  });
+  float* d = reinterpret_cast<float*>(&stage[b * LA + j]);
+  static_for<0, BH>([&](auto ii) {
+    constexpr int i = decltype(ii)::value;
+    const float px = (float)v[i] - base;
+    d[(i >> 1) * SA * 2 + (i & 1)] = (px * wx) * winy[i];
+  });
+}
+
+// Overlap-add with one lane per tile column, down all BH rows of the tile's
+// block row: the column's block(s) and synthesis weights are worked out once
+// per frame, then each row is one or two LDS reads and its store -- the
+// reference's float order of overlap_run (src/buffer.cpp:381-560).
+template <int PIX, int BW, int BH, int LSYN, int SYNW>
+__device__ __forceinline__ void overlap_cols(const PlaneJob& J, uint8_t* dst, float* seam, const float* syn,
+                                             const float* wsyn, int by, int bx0, int nblk, int RW, int stepx,
+                                             int stepy, int t) {
+  constexpr int XL = 0, XR = BW, YL = 2 * BW, YR = 2 * BW + BH;
+  using T = typename std::conditional<PIX == 1, uint8_t, typename std::conditional<PIX == 2, uint16_t, float>::type>::type;
+  const int xl = t;
+  if (xl >= RW) return;  // lanes past the tile's columns
+  const float rounder = PIX == 4 ? 0.0f : 0.5f;
+  const float fbase = (float)J.obase, norm = J.norm;
+  const int ow = J.ow, oh = J.oh, mirw = J.mirw, mirh = J.mirh, Wc = J.Wc, Hc = J.Hc, maxval = J.maxval;
+  const int interlaced = J.interlaced, half = J.Hc >> 1;
+  const long long dpitch = J.dst_pitch;
+  const int cy0 = by * stepy, cx0 = bx0 * stepx;
+  const int P = (int)J.seam_plane;
+  T* dpl = reinterpret_cast<T*>(dst + (long long)J.tc * dpitch + (long long)J.lc * sizeof(T));
+  const int pitch_el = (int)(dpitch / (long long)sizeof(T));
+  const int x = cx0 + xl - mirw;
+  const bool xin = x >= 0 && x < Wc;
+  // x class: 0 exclusive column, 1 band inside the tile, 2 band shared with
+  // the left tile (this tile holds its right block), 3 band shared with the
+  // right tile (this tile holds its left block)
+  int cls = 0, la = 0, lb = 0;
+  float wa = 0.0f, wb = 1.0f;
+  if (xl < nblk * stepx) {
+    const int bl = xl / stepx, j = xl - bl * stepx;
+    lb = bl * LSYN + j;
+    la = lb;
+    if (j < ow) {
+      wb = wsyn[XL + j];
+      if (bl >= 1) {
+        cls = 1;
+        la = (bl - 1) * LSYN + j + stepx;
+        wa = wsyn[XR + j];
+      } else {
+        cls = 2;
+      }
+    }
+  } else {
+    const int jj = xl - nblk * stepx;
+    cls = 3;
+    la = (nblk - 1) * LSYN + stepx + jj;
+    lb = la;
+    wa = wsyn[XR + jj];
+    wb = 0.0f;
+  }
+  const bool own = xin && cls <= 1, shr = xin && cls >= 2;
+  const int slot_x = (cls == 3 ? 0 : 1) * P;
+  const int xc = xin ? x : 0;
+#pragma unroll 8
+  for (int i = 0; i < BH; i++) {
+    const int ry = cy0 + i - mirh;
+    if (ry < 0 || ry >= Hc) continue;  // padding row (uniform)
+    const float h = syn[la + i * SYNW] * wa + syn[lb + i * SYNW] * wb;
+    const int yo = !interlaced ? ry : (ry < half ? 2 * ry : Hc - 1 - 2 * (ry - half));
+    const int so = yo * Wc + xc;
+    if (i >= oh && i < stepy) {  // rows of this block row only (uniform)
+      const float v = h * norm + rounder + fbase;
+      T q;
+      if constexpr (PIX == 4) {
+        const float o = v < 0.0f ? 0.0f : v;
+        q = o > 1.0f ? 1.0f : o;
+      } else {
+        int qi = (int)v;
+        qi = qi < 0 ? 0 : qi;
+        q = (T)(qi > maxval ? maxval : qi);
+      }
+      if (own) dpl[(unsigned)(yo * pitch_el + xc)] = q;
+      if (shr) seam[(unsigned)(slot_x + so)] = h;  // slot 0 = left block's part, slot 1 = right's
+    } else {
+      const bool ytop = i < oh;
+      const float w = wsyn[ytop ? YL + i : YR + i - stepy];
+      const int slot = cls <= 1 ? (ytop ? 1 : 0) : (ytop ? (cls == 3 ? 2 : 3) : (cls == 3 ? 0 : 1));
+      const float val = cls <= 1 ? h * (w * norm) : h;
+      if (xin) seam[(unsigned)(slot * P + so)] = val;
+    }
+  }
+}
+
 // ------------------------------------------------------------ spectral pass
 // Sharpen / dehalo factor of one element (src/code_impl/Sharpen.cpp:102-105);
 // for SHP_GENERAL the sharpen factor is exactly 1 when sharpen == 0, so the
@@ -313,9 +425,9 @@ template <int SHP>
 __device__ __forceinline__ float2 sharpen_s(const SpecParams& J, float2 x, const Tab& T, float gf) {
   const float g0 = gf * T.g.x, g1 = gf * T.g.y;
   const float cr = x.x - g0, ci = x.y - g1;
-  const float psd = cr * cr + ci * ci + 1.0e-15f;
+  const float psd = power(cr, ci);
   const float f = shp_factor<SHP>(J, psd, T.wsh, T.wdh);
-  return make_float2(cr * f + g0, ci * f + g1);
+  return make_float2(fmaf_(cr, f, g0), fmaf_(ci, f, g1));
 }
 
 // Apply2D (src/code_impl/Apply.cpp:13-65) with the sharpen factor folded in
@@ -324,13 +436,13 @@ template <int SHP>
 __device__ __forceinline__ float2 apply2d_s(const SpecParams& J, float2 x, const Tab& T, float gf) {
   const float g0 = gf * T.g.x, g1 = gf * T.g.y;
   const float cr = x.x - g0, ci = x.y - g1;
-  const float psd = cr * cr + ci * ci + 1e-15f;
-  float f = fast_div(psd - T.sig, psd);
+  const float psd = power(cr, ci);
+  float f = fmaf_(-T.sig, __builtin_amdgcn_rcpf(psd), 1.0f);
   f = f > J.lowlimit ? f : J.lowlimit;
   if constexpr (SHP != SHP_NONE) {
     if (!J.use_pattern) f *= shp_factor<SHP>(J, psd, T.wsh, T.wdh);
   }
-  return make_float2(cr * f + g0, ci * f + g1);
+  return make_float2(fmaf_(cr, f, g0), fmaf_(ci, f, g1));
 }
 
 // One chunk of CH rows of a lane's column: the ring spectra / Kalman state
@@ -447,7 +559,9 @@ __global__ void __launch_bounds__(NT, (run_blocks_per_cu<BH, NT, MODE, BT, SHP>(
   constexpr int CH = (MODE == MODE_W3D) ? 4 : 2;  // rows per chunk (two chunks of loads in flight)
   constexpr int NCH = CR / CH;
   static_assert(CR % CH == 0 && BH % 8 == 0 && BW % 4 == 0, "block shape");
-  static_assert(MODE != MODE_FWD, "k_run filters");
+  // MODE_FWD: the forward transforms only, frame m0 + i stored to ring slot
+  // (m0 + i) % BT -- the Wiener-3D window frames a range starts with, so
+  // every spectrum in the ring comes from this kernel's arithmetic
   // chunks of ring rows in flight ahead of the one being computed
   // (256 lanes: half the rows per lane and a register budget of 168 for
   // three wavefronts per SIMD)
@@ -513,7 +627,7 @@ __global__ void __launch_bounds__(NT, (run_blocks_per_cu<BH, NT, MODE, BT, SHP>(
     else if (J.pix == 2) gather_issue<2, RH, GQ>(src, rw, sx, npass, raw);
     else gather_issue<4, RH, GQ>(src, rw, sx, npass, raw);
   };
-  issue(L0.J, L0.src[i0], rowoff, (int)threadIdx.x);
+  if constexpr (!SPLIT) issue(L0.J, L0.src[i0], rowoff, (int)threadIdx.x);
 
 #pragma unroll 1
   for (int fi = i0; fi < i1; fi++) {
@@ -559,9 +673,16 @@ __global__ void __launch_bounds__(NT, (run_blocks_per_cu<BH, NT, MODE, BT, SHP>(
     const SpecParams SP(J);
 
     // ---- 1. window the loaded pixels into per-block row pairs
-    if (pix == 1) gather_window<1, BW, BH, NT, SA, LA, RH, GQ>(J, stage, raw, wx, wy, by, bx0, nblk, RW, stepx, t);
-    else if (pix == 2) gather_window<2, BW, BH, NT, SA, LA, RH, GQ>(J, stage, raw, wx, wy, by, bx0, nblk, RW, stepx, t);
-    else gather_window<4, BW, BH, NT, SA, LA, RH, GQ>(J, stage, raw, wx, wy, by, bx0, nblk, RW, stepx, t);
+    if constexpr (SPLIT) {
+      const uint8_t* fsrc = L.src[fi];
+      if (pix == 1) gather_cols<1, BW, BH, SA, LA>(J, stage, fsrc, ro, wx, wy, bx0, nblk, stepx, t);
+      else if (pix == 2) gather_cols<2, BW, BH, SA, LA>(J, stage, fsrc, ro, wx, wy, bx0, nblk, stepx, t);
+      else gather_cols<4, BW, BH, SA, LA>(J, stage, fsrc, ro, wx, wy, bx0, nblk, stepx, t);
+    } else {
+      if (pix == 1) gather_window<1, BW, BH, NT, SA, LA, RH, GQ>(J, stage, raw, wx, wy, by, bx0, nblk, RW, stepx, t);
+      else if (pix == 2) gather_window<2, BW, BH, NT, SA, LA, RH, GQ>(J, stage, raw, wx, wy, by, bx0, nblk, RW, stepx, t);
+      else gather_window<4, BW, BH, NT, SA, LA, RH, GQ>(J, stage, raw, wx, wy, by, bx0, nblk, RW, stepx, t);
+    }
     __syncthreads();
     if constexpr (MF) {
       // ---- 2-3. 2D r2c on the matrix cores, in rounds of one block per
@@ -640,6 +761,28 @@ __global__ void __launch_bounds__(NT, (run_blocks_per_cu<BH, NT, MODE, BT, SHP>(
     }
     __syncthreads();
 
+    if constexpr (MODE == MODE_FWD) {
+      // ---- 3. forward column transforms to the ring, then the next frame
+      float2 fc[CR];
+      if constexpr (SPLIT) {
+        static_for<0, CR>([&](auto j) { fc[j] = spec[cb * LSPEC + (2 * decltype(j)::value + hh) * SR + cc]; });
+      } else {
+        static_for<0, CR>([&](auto i) { fc[i] = spec[cb * LSPEC + decltype(i)::value * SR + cc]; });
+      }
+      __syncthreads();  // spectra read: the next frame's gather may write the stage
+      if constexpr (!SPLIT) issue(J, L.src[min(fi + 1, i1 - 1)], ro, t);
+      if constexpr (SPLIT) {
+        fft<CR, -1>(fc);
+        split_combine_fwd<BH>(fc, hh, sg);
+      } else if constexpr (!MF) {
+        fft<BH, -1>(fc);
+      }
+      if (col_on) {
+        float2* out = L.ring + (size_t)(m % BT) * L.spec_elems + colo;
+        static_for<0, CR>([&](auto i) { out[decltype(i)::value * COLS] = fc[i]; });
+      }
+      continue;
+    }
     // ---- 3. column lanes: chunk 0 in flight during the forward column FFT
     const float2* rq[NQ > 0 ? NQ : 1];
     static_for<0, NQ>([&](auto qq) {
@@ -900,14 +1043,20 @@ __global__ void __launch_bounds__(NT, (run_blocks_per_cu<BH, NT, MODE, BT, SHP>(
     // across the barrier (the registers of the row transforms are dead here;
     // the last iteration reloads its own frame: an unconditional load keeps
     // the old values from living through the iteration)
-    issue(J, L.src[min(fi + 1, i1 - 1)], ro, t);
+    if constexpr (!SPLIT) issue(J, L.src[min(fi + 1, i1 - 1)], ro, t);
     // ---- 7. overlap-add inside the tile; seam partials to this frame's seam buffer
     uint8_t* dst = L.dst[fi];
     float* seam = J.seam + (size_t)fi * 4 * (size_t)J.seam_plane;
     const float* ws = wsyn + zo;
-    if (pix == 1) overlap_run<1, BW, BH, NT, LSYN, SYNW>(J, dst, seam, syn, ws, by, bx0, nblk, RW, stepx, stepy, t);
-    else if (pix == 2) overlap_run<2, BW, BH, NT, LSYN, SYNW>(J, dst, seam, syn, ws, by, bx0, nblk, RW, stepx, stepy, t);
-    else overlap_run<4, BW, BH, NT, LSYN, SYNW>(J, dst, seam, syn, ws, by, bx0, nblk, RW, stepx, stepy, t);
+    if constexpr (SPLIT) {
+      if (pix == 1) overlap_cols<1, BW, BH, LSYN, SYNW>(J, dst, seam, syn, ws, by, bx0, nblk, RW, stepx, stepy, t);
+      else if (pix == 2) overlap_cols<2, BW, BH, LSYN, SYNW>(J, dst, seam, syn, ws, by, bx0, nblk, RW, stepx, stepy, t);
+      else overlap_cols<4, BW, BH, LSYN, SYNW>(J, dst, seam, syn, ws, by, bx0, nblk, RW, stepx, stepy, t);
+    } else {
+      if (pix == 1) overlap_run<1, BW, BH, NT, LSYN, SYNW>(J, dst, seam, syn, ws, by, bx0, nblk, RW, stepx, stepy, t);
+      else if (pix == 2) overlap_run<2, BW, BH, NT, LSYN, SYNW>(J, dst, seam, syn, ws, by, bx0, nblk, RW, stepx, stepy, t);
+      else overlap_run<4, BW, BH, NT, LSYN, SYNW>(J, dst, seam, syn, ws, by, bx0, nblk, RW, stepx, stepy, t);
+    }
     __syncthreads();  // syn read; the next frame's gather writes the stage
   }
 }
@@ -1006,6 +1155,15 @@ hipError_t launch_run_modes(int mode, int bt, int shp, const RunBatch& L, hipStr
     default: return hipErrorInvalidValue;                    \
   }
   switch (mode) {
+    case MODE_FWD:
+      switch (bt) {
+        case 2: NF3D_RUN(MODE_FWD, 2, SHP_NONE); break;
+        case 3: NF3D_RUN(MODE_FWD, 3, SHP_NONE); break;
+        case 4: NF3D_RUN(MODE_FWD, 4, SHP_NONE); break;
+        case 5: NF3D_RUN(MODE_FWD, 5, SHP_NONE); break;
+        default: return hipErrorInvalidValue;
+      }
+      break;
     case MODE_W2D: NF3D_RUN_SHP(MODE_W2D, 0); break;
     case MODE_KALMAN: NF3D_RUN_SHP(MODE_KALMAN, 0); break;
     case MODE_SHARPEN:

@@ -108,30 +108,33 @@ __device__ __forceinline__ Tab load_tab(const PlaneJob& J, int k, bool sharp, si
 }
 
 // Gains of the spectral pass use the hardware reciprocal and square root
-// (v_rcp_f32 / v_sqrt_f32, ~1 ulp): a gain is a smooth function of the power,
-// so a 1-ulp gain change moves an output pixel by ~1e-7 of its value, far
-// below the FP32 transform error the parity bar already allows.
+// (v_rcp_f32 / v_sqrt_f32, ~1 ulp) and fused multiply-adds: a gain is a
+// smooth function of the power, so a few-ulp gain change moves an output
+// pixel by ~1e-7 of its value, far below the FP32 transform error the parity
+// bar (against the FP64 evaluation of the same formulas) already allows.
 __device__ __forceinline__ float fast_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float fmaf_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+// |c|^2 + 1e-15 (the reference's psd)
+__device__ __forceinline__ float power(float re, float im) { return fmaf_(re, re, fmaf_(im, im, 1.0e-15f)); }
 
 // Factors of src/code_impl/Sharpen.cpp:102-105.
 template <class PJ>
 __device__ __forceinline__ float sharpen_factor(const PJ& J, float psd, float wsh) {
   float a = J.sharpen * wsh;
   float q = fast_div(psd * J.smax_n, (psd + J.smin_n) * (psd + J.smax_n));
-  return 1.0f + a * __builtin_amdgcn_sqrtf(q);
+  return fmaf_(a, __builtin_amdgcn_sqrtf(q), 1.0f);
 }
 template <class PJ>
 __device__ __forceinline__ float dehalo_factor(const PJ& J, float psd, float wdh) {
   float s = psd + J.ht2n;
-  return fast_div(s, s + J.dehalo * wdh * psd);
+  return fast_div(s, fmaf_(J.dehalo * wdh, psd, s));
 }
 
-__device__ __forceinline__ void wiener3d(float& re, float& im, float sig, float lowlimit) {
-  float psd = re * re + im * im + 1.0e-15f;
-  float f = fast_div(psd - sig, psd);
-  f = f > lowlimit ? f : lowlimit;
-  re *= f;
-  im *= f;
+// Wiener gain of one bin, max((psd - sig) / psd, lowlimit), as
+// max(1 - sig / psd, lowlimit) (src/code_impl/Apply.cpp:13-235).
+__device__ __forceinline__ float wiener_gain(float re, float im, float sig, float lowlimit) {
+  const float f = fmaf_(-sig, __builtin_amdgcn_rcpf(power(re, im)), 1.0f);
+  return f > lowlimit ? f : lowlimit;
 }
 
 // Apply2D (src/code_impl/Apply.cpp:13-65): one element.
@@ -142,15 +145,15 @@ __device__ __forceinline__ float2 apply2d_elem(const PlaneJob& J, float2 x, cons
     g1 = gf * T.g.y;
   }
   float cr = x.x - g0, ci = x.y - g1;
-  float psd = cr * cr + ci * ci + 1e-15f;
-  float f = fast_div(psd - T.sig, psd);
+  float psd = power(cr, ci);
+  float f = fmaf_(-T.sig, __builtin_amdgcn_rcpf(psd), 1.0f);
   f = f > J.lowlimit ? f : J.lowlimit;
   if (!J.use_pattern) {
     if (J.dehalo == 0.0f) f *= sharpen_factor(J, psd, T.wsh);
     else if (J.sharpen == 0.0f) f *= dehalo_factor(J, psd, T.wdh);
     else f *= sharpen_factor(J, psd, T.wsh) * dehalo_factor(J, psd, T.wdh);
   }
-  return make_float2(cr * f + g0, ci * f + g1);
+  return make_float2(fmaf_(cr, f, g0), fmaf_(ci, f, g1));
 }
 
 // Sharpen (src/code_impl/Sharpen.cpp:13-133): one element; caller checks
@@ -162,16 +165,16 @@ __device__ __forceinline__ float2 sharpen_elem(const PlaneJob& J, float2 x, cons
     g1 = gf * T.g.y;
   }
   float cr = x.x - g0, ci = x.y - g1;
-  float psd = cr * cr + ci * ci + 1.0e-15f;
+  float psd = power(cr, ci);
   float f;
   if (J.dehalo == 0.0f) f = sharpen_factor(J, psd, T.wsh);
   else if (J.sharpen == 0.0f) f = dehalo_factor(J, psd, T.wdh);
   else f = sharpen_factor(J, psd, T.wsh) * dehalo_factor(J, psd, T.wdh);
-  return make_float2(cr * f + g0, ci * f + g1);
+  return make_float2(fmaf_(cr, f, g0), fmaf_(ci, f, g1));
 }
 
-// Apply3D2..3D5 (src/code_impl/Apply.cpp:67-235), same float evaluation
-// order; v[0..BT-1] are the frames oldest first.
+// Apply3D2..3D5 (src/code_impl/Apply.cpp:67-235); v[0..BT-1] are the frames
+// oldest first.
 template <int BT, class PJ>
 __device__ __forceinline__ float2 apply3d_elem(const PJ& J, const float2 (&v)[5], const Tab& T, float gf) {
   const float sig = T.sig;
@@ -181,15 +184,16 @@ __device__ __forceinline__ float2 apply3d_elem(const PJ& J, const float2 (&v)[5]
     g0 = gf * T.g.x * (float)BT;
     g1 = gf * T.g.y * (float)BT;
   }
+  // the temporal DFT of the reference's formulas, the Wiener gain of each
+  // bin, and the inverse at the current frame as one fused sum per part
   float2 r;
   if constexpr (BT == 2) {
     float2 p = v[0], q = v[1];
     float f0r = q.x + p.x - g0, f0i = q.y + p.y - g1;
     float f1r = q.x - p.x, f1i = q.y - p.y;
-    wiener3d(f0r, f0i, sig, ll);
-    wiener3d(f1r, f1i, sig, ll);
-    r.x = (f0r + f1r + g0) * 0.5f;
-    r.y = (f0i + f1i + g1) * 0.5f;
+    const float w0 = wiener_gain(f0r, f0i, sig, ll), w1 = wiener_gain(f1r, f1i, sig, ll);
+    r.x = fmaf_(f0r, w0, fmaf_(f1r, w1, g0)) * 0.5f;
+    r.y = fmaf_(f0i, w0, fmaf_(f1i, w1, g1)) * 0.5f;
   } else if constexpr (BT == 3) {
     constexpr float s120 = 0.86602540378443864676372317075294f;
     constexpr float third = 1.0f / 3.0f;
@@ -198,15 +202,13 @@ __device__ __forceinline__ float2 apply3d_elem(const PJ& J, const float2 (&v)[5]
     float fcr = q.x + pnr - g0, fci = q.y + pni - g1;
     float di = s120 * (p.y - n.y);
     float dr = s120 * (n.x - p.x);
-    float fpr = q.x - 0.5f * pnr + di;
-    float fnr = q.x - 0.5f * pnr - di;
-    float fpi = q.y - 0.5f * pni + dr;
-    float fni = q.y - 0.5f * pni - dr;
-    wiener3d(fcr, fci, sig, ll);
-    wiener3d(fpr, fpi, sig, ll);
-    wiener3d(fnr, fni, sig, ll);
-    r.x = (fcr + fpr + fnr + g0) * third;
-    r.y = (fci + fpi + fni + g1) * third;
+    float mr = fmaf_(-0.5f, pnr, q.x), mi = fmaf_(-0.5f, pni, q.y);
+    float fpr = mr + di, fnr = mr - di;
+    float fpi = mi + dr, fni = mi - dr;
+    const float wc = wiener_gain(fcr, fci, sig, ll), wp = wiener_gain(fpr, fpi, sig, ll),
+                wn = wiener_gain(fnr, fni, sig, ll);
+    r.x = fmaf_(fcr, wc, fmaf_(fpr, wp, fmaf_(fnr, wn, g0))) * third;
+    r.y = fmaf_(fci, wc, fmaf_(fpi, wp, fmaf_(fni, wn, g1))) * third;
   } else if constexpr (BT == 4) {
     float2 p2 = v[0], p = v[1], q = v[2], n = v[3];
     float fp2r = (q.x + p2.x) - (p.x + n.x);
@@ -217,39 +219,35 @@ __device__ __forceinline__ float2 apply3d_elem(const PJ& J, const float2 (&v)[5]
     float fci = (q.y + p2.y) + (p.y + n.y) - g1;
     float fnr = (q.x - p2.x) - (p.y - n.y);
     float fni = (q.y - p2.y) + (p.x - n.x);
-    wiener3d(fp2r, fp2i, sig, ll);
-    wiener3d(fpr, fpi, sig, ll);
-    wiener3d(fcr, fci, sig, ll);
-    wiener3d(fnr, fni, sig, ll);
-    r.x = ((fp2r + fpr) + (fcr + fnr) + g0) * 0.25f;
-    r.y = ((fp2i + fpi) + (fci + fni) + g1) * 0.25f;
+    const float w2 = wiener_gain(fp2r, fp2i, sig, ll), wp = wiener_gain(fpr, fpi, sig, ll),
+                wc = wiener_gain(fcr, fci, sig, ll), wn = wiener_gain(fnr, fni, sig, ll);
+    r.x = fmaf_(fp2r, w2, fmaf_(fpr, wp, fmaf_(fcr, wc, fmaf_(fnr, wn, g0)))) * 0.25f;
+    r.y = fmaf_(fp2i, w2, fmaf_(fpi, wp, fmaf_(fci, wc, fmaf_(fni, wn, g1)))) * 0.25f;
   } else {
     constexpr float s72 = 0.95105651629515357211643933337938f;
     constexpr float c72 = 0.30901699437494742410229341718282f;
     constexpr float s144 = 0.58778525229247312916870595463907f;
     constexpr float c144 = -0.80901699437494742410229341718282f;
     float2 p2 = v[0], p = v[1], q = v[2], n = v[3], n2 = v[4];
-    float sum = (p2.x + n2.x) * c72 + (p.x + n.x) * c144 + q.x;
-    float dif = (-p2.y + n2.y) * s72 + (p.y - n.y) * s144;
+    float sum = fmaf_(p2.x + n2.x, c72, fmaf_(p.x + n.x, c144, q.x));
+    float dif = fmaf_(n2.y - p2.y, s72, (p.y - n.y) * s144);
     float fp2r = sum + dif, fn2r = sum - dif;
-    sum = (p2.y + n2.y) * c72 + (p.y + n.y) * c144 + q.y;
-    dif = (p2.x - n2.x) * s72 + (-p.x + n.x) * s144;
+    sum = fmaf_(p2.y + n2.y, c72, fmaf_(p.y + n.y, c144, q.y));
+    dif = fmaf_(p2.x - n2.x, s72, (n.x - p.x) * s144);
     float fp2i = sum + dif, fn2i = sum - dif;
-    sum = (p2.x + n2.x) * c144 + (p.x + n.x) * c72 + q.x;
-    dif = (p2.y - n2.y) * s144 + (p.y - n.y) * s72;
+    sum = fmaf_(p2.x + n2.x, c144, fmaf_(p.x + n.x, c72, q.x));
+    dif = fmaf_(p2.y - n2.y, s144, (p.y - n.y) * s72);
     float fpr = sum + dif, fnr = sum - dif;
-    sum = (p2.y + n2.y) * c144 + (p.y + n.y) * c72 + q.y;
-    dif = (-p2.x + n2.x) * s144 + (-p.x + n.x) * s72;
+    sum = fmaf_(p2.y + n2.y, c144, fmaf_(p.y + n.y, c72, q.y));
+    dif = fmaf_(n2.x - p2.x, s144, (n.x - p.x) * s72);
     float fpi = sum + dif, fni = sum - dif;
     float fcr = (p2.x + p.x) + q.x + (n.x + n2.x) - g0;
     float fci = (p2.y + p.y) + q.y + (n.y + n2.y) - g1;
-    wiener3d(fp2r, fp2i, sig, ll);
-    wiener3d(fpr, fpi, sig, ll);
-    wiener3d(fcr, fci, sig, ll);
-    wiener3d(fnr, fni, sig, ll);
-    wiener3d(fn2r, fn2i, sig, ll);
-    r.x = ((fp2r + fpr) + (fcr + fnr) + fn2r + g0) * 0.2f;
-    r.y = ((fp2i + fpi) + (fci + fni) + fn2i + g1) * 0.2f;
+    const float w2 = wiener_gain(fp2r, fp2i, sig, ll), wp = wiener_gain(fpr, fpi, sig, ll),
+                wc = wiener_gain(fcr, fci, sig, ll), wn = wiener_gain(fnr, fni, sig, ll),
+                wn2 = wiener_gain(fn2r, fn2i, sig, ll);
+    r.x = fmaf_(fp2r, w2, fmaf_(fpr, wp, fmaf_(fcr, wc, fmaf_(fnr, wn, fmaf_(fn2r, wn2, g0))))) * 0.2f;
+    r.y = fmaf_(fp2i, w2, fmaf_(fpi, wp, fmaf_(fci, wc, fmaf_(fni, wn, fmaf_(fn2i, wn2, g1))))) * 0.2f;
   }
   return r;
 }

@@ -167,6 +167,7 @@ typedef struct {
   float pcutoff, pfactor, sigma2, sigma3, sigma4, degrid, dehalo, hr, ht;
   int l, t, r, b, opt, ncpu, mt;
   int planes[4], nplanes_list; /* VS "planes" */
+  int preg_x, preg_y;           /* "pregion=rx[,ry]": per-region patterns (no reference counterpart) */
 } oparams;
 
 typedef struct {
@@ -195,6 +196,7 @@ typedef struct {
   int pattern_set, pattern3d_init;
   int pshow_flat;                      /* pshow has set the windows to 1 */
   double* bpsd;                        /* [nb][bh*outwidth] per-block noise PSD of the next frame, or NULL */
+  int preg_x, preg_y;                  /* per-region pattern estimation: region size in blocks (0: off) */
   cplx* grid;                          /* [bh*outwidth] */
   float sigmaSquaredNoiseNormed2D, sigmaSquaredSharpenMinNormed, sigmaSquaredSharpenMaxNormed, ht2n;
   /* Kalman state */
@@ -573,6 +575,50 @@ static void find_pattern_block(oengine* e, const cplx* X, int* px, int* py) {
     }
 }
 
+/* Per-region patterns (no reference counterpart; the generalisation of
+ * FindPatternBlock / SetPattern, src/helper.cpp:46-132): regions of
+ * preg_x x preg_y blocks each take the pattern |X - g G|^2 pwin of their own
+ * first strict minimum of the search sums over the blocks of the reference's
+ * search range inside the region, else of the plane's choice (gx, gy); every
+ * block of a region gets that pattern, as a per-block PSD. */
+static double block_sum(const oengine* e, const cplx* X, int bx, int by) {
+  int bh = e->ep.bh, ow = e->outwidth;
+  const cplx* blk = X + (size_t)(e->nox * by + bx) * bh * ow;
+  double gcur = (double)e->ep.degrid * blk[0].re / e->grid[0].re;
+  double s = 0;
+  for (int k = 0; k < bh * ow; k++) {
+    double c0 = blk[k].re - gcur * e->grid[k].re, c1 = blk[k].im - gcur * e->grid[k].im;
+    s += (c0 * c0 + c1 * c1) * e->pwin[k];
+  }
+  return s;
+}
+
+static void set_region_patterns(oengine* e, const cplx* X, int gx, int gy) {
+  int bh = e->ep.bh, ow = e->outwidth, rx = e->preg_x, ry = e->preg_y;
+  size_t n = (size_t)bh * ow;
+  if (!e->bpsd) e->bpsd = xcalloc((size_t)e->nb * n, sizeof(double));
+  for (int y0 = 0; y0 < e->noy; y0 += ry)
+    for (int x0 = 0; x0 < e->nox; x0 += rx) {
+      int x1 = x0 + rx < e->nox ? x0 + rx : e->nox, y1 = y0 + ry < e->noy ? y0 + ry : e->noy;
+      int px = -1, py = -1;
+      double best = 1e15f;
+      for (int by = y0 > 2 ? y0 : 2; by < y1 && by < e->noy - 2; by++)
+        for (int bx = x0 > 2 ? x0 : 2; bx < x1 && bx < e->nox - 2; bx++) {
+          double v = block_sum(e, X, bx, by);
+          if (v < best) { best = v; px = bx; py = by; }
+        }
+      if (px < 0) { px = gx; py = gy; }
+      const cplx* blk = X + (size_t)(e->nox * py + px) * n;
+      double gcur = (double)e->ep.degrid * blk[0].re / e->grid[0].re;
+      for (size_t k = 0; k < n; k++) {
+        double c0 = blk[k].re - gcur * e->grid[k].re, c1 = blk[k].im - gcur * e->grid[k].im;
+        double v = (c0 * c0 + c1 * c1) * e->pwin[k];
+        for (int by = y0; by < y1; by++)
+          for (int bx = x0; bx < x1; bx++) e->bpsd[((size_t)by * e->nox + bx) * n + k] = v;
+      }
+    }
+}
+
 static void set_pattern(oengine* e, const cplx* X, int px, int py) {
   int bh = e->ep.bh, ow = e->outwidth;
   const cplx* blk = X + (size_t)(e->nox * py + px) * bh * ow;
@@ -590,6 +636,8 @@ static const char* engine_init(oengine* e, const oparams* p, const oformat* f, i
   e->ep = *p;
   e->fmt = *f;
   e->frames = frames;
+  e->preg_x = p->preg_x > 0 && p->preg_y > 0 ? p->preg_x : 0;
+  e->preg_y = p->preg_x > 0 && p->preg_y > 0 ? p->preg_y : 0;
   e->chroma = f->family == 1 && plane != 0;
   e->framewidth = e->chroma ? width >> f->ssw : width;
   e->frameheight = e->chroma ? height >> f->ssh : height;
@@ -697,6 +745,7 @@ static void engine_frame(oclip* c, int pl, int n, uint8_t* dst, int dstride) {
     if (px == 0 && py == 0) find_pattern_block(e, ps, &px, &py);
     e->ep.px = px; e->ep.py = py;
     set_pattern(e, ps, px, py);
+    if (e->preg_x > 0) set_region_patterns(e, ps, px, py);
     e->pattern_set = 1;
   }
   if (pattern && !e->bpsd && e->ep.pshow) {
@@ -785,6 +834,11 @@ static void parse_args(const char* args, oparams* p, int* have_planes) {
       F(sigma2) F(sigma3) F(sigma4) F(degrid) F(dehalo) F(hr) F(ht) I(l) I(t) I(r) I(b) I(opt) I(ncpu) I(mt)
 #undef F
 #undef I
+      if (!strcmp(k, "pregion")) {
+        p->preg_x = atoi(v);
+        const char* comma = strchr(v, ',');
+        p->preg_y = comma ? atoi(comma + 1) : p->preg_x;
+      }
       if (!strcmp(k, "planes")) {
         *have_planes = 1;
         p->nplanes_list = 0;
@@ -813,13 +867,17 @@ void* nf3d_oracle_create(const char* args, int family, int sample_type, int bits
                          int width, int height, int nframes, const void* const* planes,
                          const int* strides, char* err, int errlen) {
   oclip* c = xcalloc(1, sizeof(oclip));
+  /* families 3 / 4: YUVA / RGBA, a fourth (alpha) plane at the luma size */
+  int alpha = family == 3 || family == 4;
+  if (family == 3) family = 1;
+  if (family == 4) family = 2;
   c->f.family = family;
   c->f.is_float = sample_type != 0;
   c->f.bits = bits;
   c->f.bytes = bits <= 8 ? 1 : bits <= 16 ? 2 : 4;
   c->f.ssw = family == 1 ? ssw : 0;
   c->f.ssh = family == 1 ? ssh : 0;
-  c->f.nplanes = family == 0 ? 1 : 3;
+  c->f.nplanes = family == 0 ? 1 : 3 + alpha;
   c->width = width; c->height = height; c->frames = nframes;
   c->planes = xcalloc((size_t)nframes * c->f.nplanes, sizeof(void*));
   memcpy(c->planes, planes, sizeof(void*) * (size_t)nframes * c->f.nplanes);
@@ -854,6 +912,16 @@ void* nf3d_oracle_create(const char* args, int family, int sample_type, int bits
   if (!have_planes) { p.nplanes_list = 3; p.planes[0] = 0; p.planes[1] = 1; p.planes[2] = 2; }
   for (int i = 0; i < p.nplanes_list; i++)
     if (p.planes[i] < c->f.nplanes && p.planes[i] >= 0) c->process[p.planes[i]] = 3;
+  /* An alpha plane is filtered with the chroma geometry of YUV formats
+   * (src/fft3d.hpp:186-191: IsChroma for every plane but the first), which
+   * for subsampled YUVA covers only part of the plane while the whole plane
+   * is copied out (src/fft3d.hpp:292-313): the reference leaves the rest
+   * undefined, so that case is refused. */
+  if (c->f.nplanes == 4 && c->process[3] == 3 && c->f.family == 1 && (c->f.ssw || c->f.ssh)) {
+    set_err(err, errlen, "the alpha plane of a subsampled YUVA clip cannot be filtered");
+    nf3d_oracle_free(c);
+    return NULL;
+  }
   for (int i = 0; i < c->f.nplanes; i++) {
     if (c->process[i] == 3) {
       const char* msg = engine_init(&c->eng[i], &p, &c->f, i, width, height, nframes);
@@ -912,6 +980,18 @@ int nf3d_oracle_set_block_psd(void* h, int plane, const float* psd) {
   if (!psd) { free(e->bpsd); e->bpsd = NULL; return 0; }
   if (!e->bpsd) e->bpsd = xcalloc(n, sizeof(double));
   for (size_t i = 0; i < n; i++) e->bpsd[i] = (double)psd[i];
+  return 0;
+}
+
+/* Per-region pattern estimation for this plane (rx x ry blocks; 0 = the
+ * reference's single pattern), before its first frame.  No reference
+ * counterpart. */
+int nf3d_oracle_set_pattern_regions(void* h, int plane, int rx, int ry) {
+  oclip* c = (oclip*)h;
+  if (!c->has_eng[plane] || rx < 0 || ry < 0) return -1;
+  oengine* e = &c->eng[plane];
+  e->preg_x = rx > 0 && ry > 0 ? rx : 0;
+  e->preg_y = rx > 0 && ry > 0 ? ry : 0;
   return 0;
 }
 

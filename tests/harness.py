@@ -1,11 +1,10 @@
 """Shared test helpers: small seeded clips and ctypes runners.
 
 A "runner" wraps any library that exports the FFT3D clip interface
-``<prefix>_create / <prefix>_get_frame / <prefix>_free`` with the signature of
-the reference driver (oracle/ref_driver.cpp): the FP64 oracle
-(oracle/libnf3d_oracle.so), the compiled reference (oracle/_ref/libnf3d_ref.so)
-and the product's host front end (neo_fft3d_amd/libnf3d.so, prefix
-``nf3d_clip``) all speak it, so one test can run the same clip through each.
+``<prefix>_create / <prefix>_get_frame / <prefix>_free``: the FP64 oracle
+(oracle/libnf3d_oracle.so) and the product's host front end
+(neo_fft3d_amd/libnf3d.so, prefix ``nf3d_clip``) both speak it, so one test
+can run the same clip through each.
 """
 from __future__ import annotations
 
@@ -19,9 +18,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 # NF3D_ORACLE_SO points the tests at another build of the oracle (the
 # planted-mistake checks of tests/test_oracle_mutations.py use it)
 ORACLE_SO = os.environ.get("NF3D_ORACLE_SO") or os.path.join(ROOT, "oracle", "libnf3d_oracle.so")
-REF_SO = os.path.join(ROOT, "oracle", "_ref", "libnf3d_ref.so")
 
-FAMILY = {"gray": 0, "yuv": 1, "rgb": 2}
+FAMILY = {"gray": 0, "yuv": 1, "rgb": 2, "yuva": 3, "rgba": 4}
 
 
 @dataclass
@@ -37,7 +35,7 @@ class Clip:
 
     @property
     def nplanes(self) -> int:
-        return 1 if self.family == "gray" else 3
+        return {"gray": 1, "yuv": 3, "rgb": 3, "yuva": 4, "rgba": 4}[self.family]
 
     @property
     def dtype(self):
@@ -46,7 +44,7 @@ class Clip:
         return np.uint8 if self.bits <= 8 else np.uint16
 
     def plane_shape(self, p: int):
-        if self.family == "yuv" and p > 0:
+        if self.family in ("yuv", "yuva") and p in (1, 2):
             return (self.height >> self.ssh, self.width >> self.ssw)
         return (self.height, self.width)
 
@@ -57,8 +55,8 @@ def make_clip(family="gray", bits=8, is_float=False, ssw=1, ssh=1, width=96, hei
     rng = np.random.default_rng(seed)
     maxv = 1.0 if is_float else float((1 << bits) - 1)
     scale = maxv / 255.0
-    clip = Clip(family, bits, is_float, ssw if family == "yuv" else 0,
-                ssh if family == "yuv" else 0, width, height, [])
+    yuv = family in ("yuv", "yuva")
+    clip = Clip(family, bits, is_float, ssw if yuv else 0, ssh if yuv else 0, width, height, [])
     periods = rng.uniform(6, 40, size=3)
     angles = rng.uniform(0, np.pi, size=3)
     for n in range(nframes):
@@ -163,9 +161,3 @@ def compare(a, b, is_float: bool):
 
 def oracle_runner() -> Runner:
     return Runner(ORACLE_SO, "nf3d_oracle")
-
-
-def ref_runner() -> Runner:
-    os.environ.setdefault("MKL_THREADING_LAYER", "SEQUENTIAL")
-    os.environ.setdefault("MKL_CBWR", "COMPATIBLE")
-    return Runner(REF_SO, "nf3dref")

@@ -66,7 +66,8 @@ INT_FIELDS = {"bw", "bh", "bt", "ow", "oh", "measure", "interlaced", "wintype", 
 def engine_params(args):
     from neo_fft3d_amd import default_params
     kv = dict(a.split("=") for a in args.split(";"))
-    kv.pop("dft", None)  # an engine setting, not a parameter: make_engine applies it
+    kv.pop("dft", None)  # engine settings, not parameters: make_engine applies them
+    kv.pop("pregion", None)
     p = default_params(float(kv.pop("sigma")))
     for k, v in kv.items():
         setattr(p, k, int(v) if k in INT_FIELDS else float(v))
@@ -78,6 +79,10 @@ def make_engine(args, fmt, width, height, frames, plane):
     e = Engine(engine_params(args), fmt, width, height, frames, plane)
     if "dft=1" in args.split(";"):
         e.set_dft(1)
+    kv = dict(a.split("=") for a in args.split(";"))
+    if "pregion" in kv:
+        r = [int(v) for v in kv["pregion"].split(",")]
+        e.set_pattern_regions(r[0], r[-1])
     return e
 
 
@@ -265,3 +270,18 @@ def test_graph_replays_match_eager(torch_cuda, args):
             print(f"replay {r} plane {p}: max diff {np.abs(got.astype(int) - want[p].astype(int)).max()}", flush=True)
             assert np.array_equal(got, want[p]), (r, p)
     g.close()
+
+
+@pytest.mark.parametrize("args", ["sigma=3;bt=3;bw=16;bh=16;pfactor=1;pregion=3",
+                                  "sigma=2;bt=3;bw=48;bh=48;ow=16;oh=16;pfactor=1;pregion=2,1;sharpen=0.3"])
+def test_run_device_region_patterns(torch_cuda, oracle, args):
+    """Per-region patterns through the frame-loop kernel (one pattern per
+    block) against the oracle."""
+    clip = make_clip("gray", 8, False, width=320, height=200, nframes=24, seed=17)
+    from harness import Runner
+    got = run_range(torch_cuda, clip, args, [24])
+    ho = oracle.open(args, clip)
+    for n in range(24):
+        (mx, fe, rms), = compare([got[n]], [ho.get(n)[0]], False)
+        print(f"{args} n={n} max={mx:.3g} exact={fe:.5f} rms={rms:.3g}", flush=True)
+        assert mx <= 1 and fe >= 0.999, (n, mx, fe)

@@ -1,62 +1,21 @@
 """The FP64 oracle, pinned before it is trusted (CPU only).
 
-1. Against the reference's own outputs: golden vectors written by
-   tests/golden/make_golden.py from the compiled reference C path (opt=1),
-   covering every mode (bt -1..5, Kalman, pattern estimation and px/py,
-   sigma2..4 patterns, degrid off, window types 1/2, beta, interlaced, block
-   shapes 16..64 and non-square, YUV crop, 10/16-bit, float RGB) and the
-   Gray8 640x360 bt=3 48/16 workload.  Bar: integer +-1 LSB and >= 99.9%
-   exact (>= 99% for full-range 16-bit content, where FP32 rounding of the
-   reference itself resolves the 1/65535 boundary less finely), float 1e-4.
-2. Against closed forms that a plausible restatement mistake would break:
-   the mixed-radix DFT vs a brute-force DFT; sigma = 0 reconstructs the input
-   exactly (the analysis x synthesis windows form a partition of unity for
-   every wintype); a constant frame stays constant.  The gains themselves are
-   pinned by closed forms in test_oracle_closed_form.py.
+Closed forms that a plausible restatement mistake would break: the
+mixed-radix DFT vs a brute-force DFT; sigma = 0 reconstructs the input exactly
+(the analysis x synthesis windows form a partition of unity for every
+wintype); a constant frame stays constant; a static clip under a huge sigma
+goes to its floor.  The gains themselves are pinned by closed forms in
+test_oracle_closed_form.py, and test_oracle_mutations.py checks that planted
+mistakes fail those.  The reference cannot be compiled here (its FFT backend,
+FFTW, is absent), so no reference outputs are used.
 """
 import ctypes
-import glob
-import json
 import os
 
 import numpy as np
 import pytest
 
 from harness import ORACLE_SO, Clip, compare, make_clip, oracle_runner
-
-GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
-
-
-def load_golden(path):
-    z = np.load(path, allow_pickle=False)
-    meta = json.loads(str(z["meta"]))
-    nplanes = 1 if meta["family"] == "gray" else 3
-    nframes = z["in0"].shape[0]
-    frames = [[np.ascontiguousarray(z[f"in{p}"][n]) for p in range(nplanes)] for n in range(nframes)]
-    outs = [[z[f"out{p}"][n] for p in range(nplanes)] for n in range(nframes)]
-    clip = Clip(meta["family"], meta["bits"], meta["is_float"], meta["ssw"], meta["ssh"], meta["width"],
-                meta["height"], frames)
-    return clip, meta["args"], outs
-
-
-def golden_bar(clip):
-    if clip.is_float:
-        return None
-    return 0.99 if clip.bits > 12 else 0.999
-
-
-@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p)[:-4] for p in GOLDEN])
-def test_oracle_matches_reference_golden(path):
-    clip, args, outs = load_golden(path)
-    h = oracle_runner().open(args, clip)
-    bar = golden_bar(clip)
-    for n, want in enumerate(outs):
-        got = h.get(n)
-        for p, (mx, exact, rms) in enumerate(compare(got, want, clip.is_float)):
-            if clip.is_float:
-                assert mx <= 1e-4, (n, p, mx)
-            else:
-                assert mx <= 1 and exact >= bar, (n, p, mx, exact)
 
 
 def test_dft_matches_brute_force():

@@ -452,3 +452,85 @@ def test_interlaced_fields_filtered_separately():
     assert checked >= hh // 3
     plain = oracle_runner().open(args(sigma=sigma, bt=1, degrid=1, interlaced=0), fclip([img]))
     assert np.max(np.abs(out0(plain, 0) - got)) > 1e-3
+
+
+# -------------------------------------------------------------------- alpha
+@pytest.mark.parametrize("family,ss", [("yuva", 1), ("yuva", 0), ("rgba", 0)])
+def test_alpha_plane_copied_unless_named(family, ss):
+    """A fourth plane is copied by default (src/fft3d.hpp:149-169: it is
+    never in the default plane list); named in `planes`, it is filtered with
+    the chroma geometry and centring for YUV (:186-191) -- here 4:4:4, where
+    that is the whole plane -- and like luma for RGB: the lower-limit check
+    gives 128 + (x - 128)/2 and x/2."""
+    clip = make_clip(family, 8, False, ssw=ss, ssh=ss, width=96, height=64, nframes=3, seed=6, noise=20)
+    h = oracle_runner().open("sigma=1e5;beta=2;bt=3;bw=16;bh=16;degrid=0", clip)
+    for n in range(3):
+        assert np.array_equal(h.get(n)[3], clip.frames[n][3])
+    if family == "yuva" and ss:
+        from harness import FilterError
+        with pytest.raises(FilterError, match="alpha plane of a subsampled YUVA"):
+            oracle_runner().open("sigma=1e5;beta=2;bt=3;bw=16;bh=16;planes=0,1,2,3", clip)
+        return
+    h = oracle_runner().open("sigma=1e5;beta=2;bt=3;bw=16;bh=16;degrid=0;planes=0,1,2,3", clip)
+    base = 128.0 if family == "yuva" else 0.0
+    for n in range(3):
+        got = h.get(n)[3]
+        x = clip.frames[n][3].astype(np.float64)
+        want = np.trunc(0.5 * (x - base) + 0.5 + base)
+        even = (clip.frames[n][3] % 2) == 0
+        d = np.abs(got - want)
+        assert d.max() <= 1 and np.all(d[even] == 0), (n, d.max())
+
+
+# --------------------------------------------------- per-region patterns
+def _set_regions(h, plane, rx, ry):
+    lib = ctypes.CDLL(ORACLE_SO)
+    lib.nf3d_oracle_set_pattern_regions.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    assert lib.nf3d_oracle_set_pattern_regions(ctypes.c_void_p(h.h), plane, rx, ry) == 0
+
+
+def test_region_patterns_whole_grid_equal_global():
+    clip = make_clip("gray", 8, False, width=160, height=128, nframes=5, seed=14, noise=10)
+    args = "sigma=3;bt=3;bw=16;bh=16;pfactor=1"
+    hg = oracle_runner().open(args, clip)
+    hr = oracle_runner().open(args, clip)
+    _set_regions(hr, 0, 100, 100)
+    for n in range(5):
+        (mx, fe, _), = compare(hr.get(n), hg.get(n), False)
+        assert mx == 0, n
+
+
+def test_region_patterns_each_region_its_own_flat_block():
+    """Two regions (left / right half of the block grid), each with one
+    planted low-amplitude block: blocks of each region are filtered with that
+    region's pattern, W = 1 - (A'/A)^2 pwin_k at the cosine bin."""
+    M, A, pcut = 0.5, 0.2, 0.1
+    w, hh = 193, 97   # w - 1 multiple of P/2
+    img = stripes(M, A, w=w, h=hh)
+    plant = {(4, 4): 0.1, (13, 5): 0.06}
+    for (bx, by), a2 in plant.items():
+        ys, xs = block_region(bx, by)
+        img[ys, xs] = stripes(M, a2, w=w, h=hh)[ys, xs]
+    h = oracle_runner().open(args(sigma=2, bt=1, pfactor=1, pcutoff=pcut, degrid=1), fclip([img]))
+    nox = (w - OW + STEP - 1) // STEP + 2
+    _set_regions(h, 0, 9, 100)
+    got = out0(h, 0)
+    fw = 2 * KX / BW
+    pwin = fw * fw / (fw * fw + pcut * pcut)
+    x = np.arange(w)
+    checked = 0
+    for (bx, by), a2 in plant.items():
+        region = range(0, 9) if bx < 9 else range(9, nox)
+        want = M + A * (1 - (a2 / A) ** 2 * pwin) * np.cos(2 * np.pi * x / P)
+        m = far_mask(bx, by, img.shape)
+        # pixels all of whose blocks lie in this region
+        cols = np.zeros(w, bool)
+        for c in range(w):
+            cx = c + STEP
+            bxs = [b for b in range(nox) if b * STEP <= cx < b * STEP + BW]
+            cols[c] = all(b in region for b in bxs)
+        m &= cols[None, :]
+        assert m.sum() > 1000
+        assert np.max(np.abs(got - want[None, :])[m]) < TOL, (bx, by)
+        checked += 1
+    assert checked == 2
