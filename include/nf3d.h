@@ -190,6 +190,17 @@ int nf3d_gather_rows(const void* src, const int* idx, void* dst, int rows, int r
 long long nf3d_mlp_work_bytes(const nf3d_mlp* m, int n);
 int nf3d_mlp_predict(nf3d_mlp* m, const void* x, int n, float* out, int exp_out, float scale, void* work,
                      void* stream, char* err, size_t errlen);
+/* nf3d_mlp_predict without a workspace and without any allocation: the
+ * hidden activations live in a fixed array the library holds on each GPU
+ * (16 MiB: n * n_hidden <= 4,194,304, i.e. 16384 rows at 256 hidden units;
+ * larger n is refused with an error before any launch).  No counterpart in
+ * the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, hosts that forbid hipMalloc).  Every
+ * call on one GPU shares that GPU's array, so calls that overlap on one GPU
+ * (different streams or threads without ordering) race; each call writes
+ * every element it reads, so nothing carries over between calls. */
+int nf3d_mlp_predict_noalloc(nf3d_mlp* m, const void* x, int n, float* out, int exp_out, float scale, void* stream,
+                             char* err, size_t errlen);
 /* Gradients of the Whittle loss mean(y + t e^-y) (y = predicted log-PSD,
  * t = periodogram targets [n][n_out] FP32) over a batch of 1..max_batch rows;
  * `loss` (optional) waits for the stream.  train_step adds an Adam update

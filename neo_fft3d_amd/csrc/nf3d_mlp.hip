@@ -729,6 +729,62 @@ int nf3d_mlp_predict(nf3d_mlp* m, const void* x, int n, float* out, int exp_out,
   });
 }
 
+}  // extern "C"
+
+// ------------------------------------------------- inference without allocation
+// The hidden activations of nf3d_mlp_predict_noalloc live in this array
+// instead of a caller's workspace: 2 layers x up to 16384 rows x 256 hidden
+// units, BF16 (16 MiB), one copy per GPU, held until the process ends.  Host
+// code never passes `scratch` itself to a kernel (on the host the name is a
+// stand-in, not a device address): it looks up each device's address once
+// with hipGetSymbolAddress and passes that.
+namespace {
+constexpr long long kScratchElems = 2LL * 16384 * 256;
+constexpr int kMaxDevices = 64;
+}  // namespace
+static __device__ __align__(256) __bf16 scratch[kScratchElems];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `dev`'s copy of the scratch array, looked up on the
+// first call on that device with the device current.
+void* scratch_on(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) throw MlpFail{"device index outside 0..63"};
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int cur = 0;
+    mcheck(hipGetDevice(&cur), "hipGetDevice");
+    mcheck(hipSetDevice(dev), "hipSetDevice");
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    (void)hipSetDevice(cur);
+    if (e != hipSuccess || !ptr) {
+      (void)hipGetLastError();
+      throw MlpFail{std::string("hipGetSymbolAddress of the scratch array: ") + hipGetErrorString(e)};
+    }
+    g_scratch_ptr[dev] = ptr;
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace
+
+extern "C" {
+
+int nf3d_mlp_predict_noalloc(nf3d_mlp* m, const void* x, int n, float* out, int exp_out, float scale, void* stream,
+                             char* err, size_t errlen) {
+  return guarded(err, errlen, [&] {
+    if (!m || n < 0 || (n > 0 && (!x || !out))) throw MlpFail{"bad arguments"};
+    if (2LL * n * m->nh > kScratchElems)
+      throw MlpFail{"no-allocation inference holds at most 16384 rows of 256 hidden units (n * n_hidden <= " +
+                    std::to_string(kScratchElems / 2) + ")"};
+    if (!n) return;
+    __bf16* w1 = static_cast<__bf16*>(scratch_on(m->device));
+    m->predict(static_cast<const __bf16*>(x), n, out, exp_out, scale, w1, w1 + (size_t)n * m->nh, st(stream));
+  });
+}
+
 int nf3d_mlp_grad(nf3d_mlp* m, const void* x, const float* t, int n, double* loss, void* stream, char* err,
                   size_t errlen) {
   std::lock_guard<std::mutex> lock(m->mu_lock);
