@@ -215,6 +215,15 @@ int nf3d_mlp_train_step(nf3d_mlp* m, const void* x, const float* t, int n, float
  * computes it) of one device plane into compact spectra [nb][bh][bw/2+1]. */
 int nf3d_engine_spectra(nf3d_engine* e, const void* src, int src_stride, void* spec_out, void* stream, char* err,
                         size_t errlen);
+/* The per-region noise-pattern estimate of one device plane as a per-block
+ * PSD (no reference counterpart; the estimator of
+ * nf3d_engine_set_pattern_regions applied to `src`, whatever pfactor is):
+ * out[nb][bh][bw/2+1] (device, FP32) holds, for every block, its rx x ry
+ * region's pattern |X - degrid correction|^2 x pattern window
+ * (src/helper.cpp:92-132).  A second label source for the learned PSD: it
+ * needs no knowledge of the noise.  The engine's own pattern is untouched. */
+int nf3d_engine_region_psd(nf3d_engine* e, const void* src, int src_stride, int rx, int ry, float* out,
+                           void* stream, char* err, size_t errlen);
 /* Use the model's per-block PSD (times `strength`) as the noise pattern of
  * every frame (null model: back to the reference's sigma / pattern).  The
  * model must be n_in = n_out = bh * (bw/2+1) and must not be trained while

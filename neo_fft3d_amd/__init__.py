@@ -168,6 +168,18 @@ class Engine:
         if L.nf3d_engine_set_pattern_regions(self.h, rx, ry, err, 256) != 0:
             raise Nf3dError(err.value.decode())
 
+    def region_psd(self, src_ptr: int, src_stride: int, rx: int, ry: int, out_ptr: int, stream: int = 0):
+        """Per-region pattern estimate of one device plane as a per-block PSD
+        written to out_ptr ([nb][bh][bw/2+1] float32, device)."""
+        L = lib()
+        L.nf3d_engine_region_psd.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p,
+                                             ctypes.c_size_t]
+        err = ctypes.create_string_buffer(256)
+        if L.nf3d_engine_region_psd(self.h, ctypes.c_void_p(src_ptr), src_stride, rx, ry, ctypes.c_void_p(out_ptr),
+                                    ctypes.c_void_p(stream), err, 256) != 0:
+            raise Nf3dError(err.value.decode())
+
     def set_dft(self, mode: int):
         """2D transform form: 0 = FFT in registers, 1 = DFT-matrix products on
         the matrix cores (32x32 blocks)."""

@@ -162,6 +162,11 @@ struct nf3d_engine {
   float2* d_spec_tmp = nullptr;
   float* d_psums = nullptr;
   int* d_choice = nullptr;
+  // region_psd scratch (kept apart from the engine's own pattern estimate)
+  float2* d_rspec = nullptr;
+  float* d_rsums = nullptr;
+  int* d_rchoice = nullptr;
+  float* d_rglobal = nullptr;
   // staging for host-fed planes
   std::vector<uint8_t*> d_stage;
   std::vector<int> stage_frame;
@@ -226,6 +231,10 @@ struct nf3d_engine {
     hipFree(d_lwork);
     hipFree(d_rpat2d);
     hipFree(d_rpat3d);
+    hipFree(d_rspec);
+    hipFree(d_rsums);
+    hipFree(d_rchoice);
+    hipFree(d_rglobal);
     for (auto* p : d_stage) hipFree(p);
     hipFree(d_dst);
     for (auto ev : ev_pool) hipEventDestroy(ev);
@@ -726,6 +735,42 @@ struct nf3d_engine {
     ep.px = choice[0];
     ep.py = choice[1];
     pattern_set = true;
+  }
+
+  // The per-region noise patterns of one plane as a per-block PSD: the
+  // plane's analysis, the plane-wide search (the fallback of regions without
+  // a searchable block), then each region's own search and its block's
+  // |X - degrid correction|^2 x pattern window written to every block of the
+  // region (src/helper.cpp:46-132, region by region).  Leaves the engine's
+  // own pattern untouched.
+  void region_psd(const uint8_t* src, long long pitch, int rx, int ry, float* out, hipStream_t s) {
+    if (!d_rspec) {
+      d_rspec = dev_alloc<float2>(spec_elems);
+      d_rsums = dev_alloc<float>(nb);
+      d_rchoice = dev_alloc<int>(2);
+      d_rglobal = dev_alloc<float>((size_t)ep.bh * outwidth);
+    }
+    PlaneJob J = base_job();
+    J.src = src;
+    J.src_pitch = pitch;
+    J.spec_out = d_rspec;
+    launch(nf3d::MODE_FWD, 0, J, s);
+    PatternJob P{};
+    P.nox = nox;
+    P.noy = noy;
+    P.bh = ep.bh;
+    P.outwidth = outwidth;
+    P.spec = d_rspec;
+    P.grid = d_grid;
+    P.pwin = d_pwin;
+    P.degrid = ep.degrid;
+    P.px = 0;
+    P.py = 0;
+    P.pattern2d = d_rglobal;
+    P.sums = d_rsums;
+    P.choice = d_rchoice;
+    check(nf3d::launch_pattern(P, s), "pattern kernels");
+    check(nf3d::launch_pattern_regions(P, rx, ry, out, out, 1.0f, s), "region pattern kernel");
   }
 
   void ensure_pattern3d(int btcur, hipStream_t s) {
@@ -1658,6 +1703,24 @@ int nf3d_engine_spectra(nf3d_engine* e, const void* src, int src_stride, void* s
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : e->stream;
     e->order_before(s);
     e->launch(nf3d::MODE_FWD, 0, F, s);
+    e->order_after(s);
+  } catch (const Failure& f) {
+    copy_err(err, errlen, f.msg);
+    return -1;
+  }
+  return 0;
+}
+
+int nf3d_engine_region_psd(nf3d_engine* e, const void* src, int src_stride, int rx, int ry, float* out,
+                           void* stream, char* err, size_t errlen) {
+  std::lock_guard<std::mutex> lock(e->mu);
+  try {
+    if (!src || !out) fail("null plane or output");
+    if (rx < 1 || ry < 1) fail("pattern regions: both sizes >= 1 blocks");
+    check(hipSetDevice(e->device), "hipSetDevice");
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : e->stream;
+    e->order_before(s);
+    e->region_psd(static_cast<const uint8_t*>(src), src_stride, rx, ry, out, s);
     e->order_after(s);
   } catch (const Failure& f) {
     copy_err(err, errlen, f.msg);

@@ -14,8 +14,12 @@ them and generates the training data:
                   variance, times a Perlin sigma field in [0.005, 0.04]
   features        the filter's own analysis (nf3d_engine_spectra: cover,
                   window, r2c) of the noisy plane, ln(|c|^2 + eps) normalised
-  targets         the same analysis of the noise alone: its periodogram, whose
-                  expectation is the block's noise PSD
+  targets         labels="noise": the same analysis of the noise alone, its
+                  periodogram, whose expectation is the block's noise PSD;
+                  labels="regions": the method's own pattern estimator on the
+                  noisy plane (nf3d_engine_region_psd: each region's flattest
+                  block, |X - degrid correction|^2 x pattern window), which
+                  needs no knowledge of the noise
   loss            Whittle: mean(y + t e^-y), minimised by y = ln E[t]
 
 Device memory is held in torch tensors (allocation only); every computation on
@@ -191,6 +195,16 @@ def engine_spectra(engine: Engine, src_ptr: int, stride: int, spec_ptr: int, str
         engine.sync()
 
 
+def region_labels(engine: Engine, src_ptr: int, stride: int, rx: int, ry: int, out_ptr: int):
+    """The engine's per-region pattern estimate of one device plane as
+    per-block PSD targets [nb][bh*(bw/2+1)] (nf3d_engine_region_psd), with
+    the same waits as engine_spectra on stream 0."""
+    import torch
+    torch.cuda.synchronize()
+    engine.region_psd(src_ptr, stride, rx, ry, out_ptr)
+    engine.sync()
+
+
 def set_learned(engine: Engine, mlp: Mlp | None, strength: float = 1.0):
     err = ctypes.create_string_buffer(512)
     if _bind().nf3d_engine_set_learned(engine.h, mlp.h if mlp is not None else None, ctypes.c_float(strength),
@@ -300,11 +314,15 @@ class Dataset:
 
 
 def build_dataset(mlp: Mlp, clips: int, width: int, height: int, frames: int, seed: int, eps: float = 1e-6,
-                  log=None, bw: int = BW, ow: int = OW, scale: float = 1.0) -> Dataset:
+                  log=None, bw: int = BW, ow: int = OW, scale: float = 1.0, labels: str = "noise",
+                  regions: tuple = (4, 4)) -> Dataset:
     """Blocks of `clips` seeded clips (3 planes x `frames` frames each) through
     the filter's analysis (bw x bw blocks, overlap ow), samples in units of
-    `scale`.  Sets the model's input normalisation from the first clip's
-    features."""
+    `scale`, with targets from `labels` ("noise" or "regions" of
+    regions[0] x regions[1] blocks).  Sets the model's input normalisation
+    from the first clip's features."""
+    if labels not in ("noise", "regions"):
+        raise ValueError("labels: 'noise' or 'regions'")
     import torch
     eng = analysis_engine(width, height, bw=bw, ow=ow)
     g = eng.geometry()
@@ -331,9 +349,15 @@ def build_dataset(mlp: Mlp, clips: int, width: int, height: int, frames: int, se
                     engine_spectra(eng, planes_noisy[p, f].data_ptr(), stride, spec.data_ptr())
                     s = spec.cpu().numpy().astype(np.float64)
                     lp.append(np.log(s[..., 0] ** 2 + s[..., 1] ** 2 + eps))
-                    engine_spectra(eng, planes_noise[p, f].data_ptr(), stride, spec.data_ptr())
-                    s = spec.cpu().numpy().astype(np.float64)
-                    lt.append(np.log(s[..., 0] ** 2 + s[..., 1] ** 2 + 1e-30))
+                    if labels == "noise":
+                        engine_spectra(eng, planes_noise[p, f].data_ptr(), stride, spec.data_ptr())
+                        s = spec.cpu().numpy().astype(np.float64)
+                        lt.append(np.log(s[..., 0] ** 2 + s[..., 1] ** 2 + 1e-30))
+                    else:
+                        tr = torch.empty((nb, d), dtype=torch.float32, device="cuda")
+                        region_labels(eng, planes_noisy[p, f].data_ptr(), stride, regions[0], regions[1],
+                                      tr.data_ptr())
+                        lt.append(np.log(tr.cpu().numpy().astype(np.float64) + 1e-30))
             lp = np.concatenate(lp)
             lt = np.concatenate(lt)
             mu = lp.mean(0)
@@ -344,8 +368,12 @@ def build_dataset(mlp: Mlp, clips: int, width: int, height: int, frames: int, se
             for f in range(frames):
                 engine_spectra(eng, planes_noisy[p, f].data_ptr(), stride, spec.data_ptr())
                 mlp.features(spec.data_ptr(), nb, X[row].data_ptr())
-                engine_spectra(eng, planes_noise[p, f].data_ptr(), stride, spec.data_ptr())
-                periodogram(spec.data_ptr(), T[row].data_ptr(), nb * d)
+                if labels == "noise":
+                    engine_spectra(eng, planes_noise[p, f].data_ptr(), stride, spec.data_ptr())
+                    periodogram(spec.data_ptr(), T[row].data_ptr(), nb * d)
+                else:
+                    region_labels(eng, planes_noisy[p, f].data_ptr(), stride, regions[0], regions[1],
+                                  T[row].data_ptr())
                 row += nb
         if log and (c + 1) % max(1, clips // 4) == 0:
             log(f"dataset: {c + 1}/{clips} clips, {row} blocks")
@@ -393,13 +421,15 @@ def train(mlp: Mlp, data: Dataset, steps: int, batch: int = 4096, lr: float = 1e
 
 def train_model(clips: int = 48, frames: int = 2, width: int = 384, height: int = 384, steps: int = 1500,
                 batch: int = 4096, hidden: int = 256, lr: float = 1e-3, seed: int = 0x4C50, log=None,
-                bw: int = BW, ow: int = OW, scale: float = 1.0):
+                bw: int = BW, ow: int = OW, scale: float = 1.0, labels: str = "noise", regions: tuple = (4, 4)):
     """Model trained from scratch on seeded data for bw x bw blocks with
     overlap ow, samples in units of `scale` (1: float planes in [0, 1]; 255:
-    8-bit; 1023: 10-bit; 65535: 16-bit); returns (mlp, dataset, losses)."""
+    8-bit; 1023: 10-bit; 65535: 16-bit), targets from `labels` (see
+    build_dataset); returns (mlp, dataset, losses)."""
     d = bw * (bw // 2 + 1)
     mlp = Mlp(d, hidden, d, batch)
-    data = build_dataset(mlp, clips, width, height, frames, seed, log=log, bw=bw, ow=ow, scale=scale)
+    data = build_dataset(mlp, clips, width, height, frames, seed, log=log, bw=bw, ow=ow, scale=scale,
+                         labels=labels, regions=regions)
     mlp.set_params(init_params(d, hidden, d, seed, out_bias=data.out_bias))
     losses = train(mlp, data, steps, batch, lr, seed, log=log)
     return mlp, data, losses

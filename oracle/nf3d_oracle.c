@@ -995,6 +995,18 @@ int nf3d_oracle_set_pattern_regions(void* h, int plane, int rx, int ry) {
   return 0;
 }
 
+/* The per-block PSD the plane filters with (set by nf3d_oracle_set_block_psd
+ * or estimated per region), [nb][bh*outwidth]; -1 when there is none.  No
+ * reference counterpart. */
+int nf3d_oracle_block_psd_get(void* h, int plane, double* out) {
+  oclip* c = (oclip*)h;
+  if (!c->has_eng[plane]) return -1;
+  oengine* e = &c->eng[plane];
+  if (!e->bpsd) return -1;
+  memcpy(out, e->bpsd, (size_t)e->nb * e->ep.bh * e->outwidth * sizeof(double));
+  return 0;
+}
+
 /* Per-plane geometry for tests: nox, noy, outwidth, mirw, mirh. */
 int nf3d_oracle_geometry(void* h, int plane, int* out5) {
   oclip* c = (oclip*)h;

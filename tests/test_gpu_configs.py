@@ -196,3 +196,121 @@ def test_mixed_clip_set_one_batch(torch_cuda):
     for k, c in enumerate(clips):
         exact = 0.99 if c.clip.bits == 16 else 0.999
         compare_with_oracle(f"mix {k} {c.args} learned={MIX[k][-1]}", c.clip, c.args, c.outputs(), exact=exact)
+
+
+# ------------------------------------------------------------ serving form
+S = 15  # staging slots per plane: a multiple of every bt, longer than any window
+
+
+def random_psd_model(bw, bits, sigma, seed):
+    """A network with random weights around ln(sigma_s^2 bw bh): a PSD that
+    varies from block to block."""
+    from neo_fft3d_amd.learned import Mlp, init_params
+    n = bw * (bw // 2 + 1)
+    m = Mlp(n, 64, n, 64)
+    bias = np.full(n, np.log((sigma * 2.0 ** (bits - 8)) ** 2 * bw * bw), np.float32)
+    m.set_params(init_params(n, 64, n, seed, out_bias=bias))
+    m.set_input_norm(np.full(n, 8.0, np.float32), np.full(n, 0.3, np.float32), 1e-6)
+    return m
+
+
+class StagedClip:
+    """One clip of the mixed set served frame by frame: frame j of each plane
+    is copied into staging slot j % S when it arrives and filtered into
+    output slot j % S, so the engines see the clip's frames at fixed
+    addresses that repeat every S frames."""
+
+    def __init__(self, torch, args, clip, model):
+        from neo_fft3d_amd import Engine, make_format
+        from neo_fft3d_amd.learned import set_learned
+        F = len(clip.frames)
+        fmt = make_format("yuv", clip.bits, False, clip.ssw, clip.ssh)
+        self.res, self.stg, self.out, self.engines = [], [], [], []
+        for p in range(3):
+            host = np.stack([clip.frames[n][p] for n in range(F)])
+            r = torch.from_numpy(host).cuda()
+            self.res.append(r)
+            self.stg.append(torch.zeros((S,) + tuple(r.shape[1:]), dtype=r.dtype, device="cuda"))
+            self.out.append(torch.zeros_like(self.stg[-1]))
+            e = Engine(engine_params(args), fmt, clip.width, clip.height, F, p)
+            if model is not None:
+                set_learned(e, model, 1.0)
+            self.engines.append(e)
+        self.F = F
+
+    def arrive(self, j):
+        for p in range(3):
+            self.stg[p][j % S].copy_(self.res[p][j])
+
+    def items(self, k):
+        out = []
+        for p, e in enumerate(self.engines):
+            st = self.stg[p][0].stride(0) * self.stg[p].element_size()
+            out.append((e, k, 1, [self.stg[p][j % S].data_ptr() for j in range(self.F)], st,
+                        [self.out[p][j % S].data_ptr() for j in range(self.F)], st))
+        return out
+
+
+def test_mixed_set_serving_graphs_bit_exact(torch_cuda):
+    """The mixed clip set served one frame step at a time: (a) one batched
+    call per step, (b) per-clip calls per step, (c) after two eager steps, one
+    HIP graph per phase k mod S captured from the batched call of that step
+    (each replayed once at capture to advance the ring), then replayed for
+    the following periods.  Every output frame of (b) and (c) equals (a) bit
+    for bit; the learned clips use networks whose PSD varies per block."""
+    from neo_fft3d_amd import Graph, run_device_batch
+    from neo_fft3d_amd.synth import SEED_BASE
+    torch = torch_cuda
+    F = 40
+    sets = {"batch": [], "single": [], "graph": []}
+    models = []
+    for k, (w, h, ss, bits, bw, ow, bt, sigma, learned) in enumerate(MIX):
+        scale = 2.0 ** (bits - 8)
+        clip = rects(w, h, F, SEED_BASE + 30 + k, ssw=ss, ssh=ss, bits=bits, noise_luma=3.0 * scale,
+                     noise_chroma=2.0 * scale)
+        args = f"sigma={sigma};bt={bt};bw={bw};bh={bw};ow={ow};oh={ow}"
+        m = random_psd_model(bw, bits, sigma, 100 + k) if learned else None
+        models.append(m)
+        for key in sets:
+            sets[key].append(StagedClip(torch, args, clip, m))
+    stream = torch.cuda.Stream()
+    sp = stream.cuda_stream
+    graphs = {}
+    warm = 3  # the first frame-loop step (bt=5: frame 2) fills the ring eagerly
+    checked = 0
+    with torch.cuda.stream(stream):
+        for j in range(3):
+            for key in sets:
+                for c in sets[key]:
+                    c.arrive(j)
+        for k in range(F - 2):
+            if k > 0:
+                for key in sets:
+                    for c in sets[key]:
+                        c.arrive(k + 2)
+            run_device_batch([it for c in sets["batch"] for it in c.items(k)], sp)
+            for c in sets["single"]:
+                for it in c.items(k):
+                    run_device_batch([it], sp)
+            items = [it for c in sets["graph"] for it in c.items(k)]
+            if k < warm:
+                run_device_batch(items, sp)
+            else:
+                if k < warm + S:
+                    print(f"capture phase {k % S} at frame {k}", flush=True)
+                    graphs[k % S] = Graph(items, sp)
+                graphs[k % S].launch(sp)
+            stream.synchronize()
+            for a, b, g in zip(sets["batch"], sets["single"], sets["graph"]):
+                for p in range(3):
+                    want = a.out[p][k % S]
+                    assert torch.equal(b.out[p][k % S], want), ("single", k, p)
+                    assert torch.equal(g.out[p][k % S], want), ("graph", k, p)
+                    checked += 1
+    print(f"{checked} plane-frames equal; {len(graphs)} graphs, replayed over {F - 2 - warm} steps", flush=True)
+    for g in graphs.values():
+        g.close()
+    for key in sets:
+        for c in sets[key]:
+            for e in c.engines:
+                e.close()

@@ -272,6 +272,43 @@ def test_graph_replays_match_eager(torch_cuda, args):
     g.close()
 
 
+@pytest.mark.parametrize("bw,ow,rx,ry", [(16, 4, 3, 3), (48, 16, 2, 1), (32, 10, 100, 100)])
+def test_region_psd_matches_oracle(torch_cuda, oracle, bw, ow, rx, ry):
+    """nf3d_engine_region_psd (the learned PSD's second label source) against
+    the oracle's per-region patterns of the same plane (estimated on frame
+    pframe = 0): FP32 spectra against FP64, <= 1e-4 of the largest value; one
+    region spanning the grid gives the plane's single pattern everywhere."""
+    import ctypes
+    from harness import ORACLE_SO
+    from neo_fft3d_amd import make_format
+    clip = make_clip("gray", 8, False, width=320, height=200, nframes=3, seed=17)
+    args = f"sigma=2;bt=1;bw={bw};bh={bw};ow={ow};oh={ow};pfactor=1;pregion={rx},{ry}"
+    ho = oracle.open(args, clip)
+    ho.get(0)
+    lib = ctypes.CDLL(ORACLE_SO)
+    geo = (ctypes.c_int * 5)()
+    lib.nf3d_oracle_geometry.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    assert lib.nf3d_oracle_geometry(ctypes.c_void_p(ho.h), 0, geo) == 0
+    nb, n = geo[0] * geo[1], bw * geo[2]
+    want = np.zeros(nb * n)
+    lib.nf3d_oracle_block_psd_get.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    assert lib.nf3d_oracle_block_psd_get(ctypes.c_void_p(ho.h), 0, want.ctypes.data_as(ctypes.c_void_p)) == 0
+    e = make_engine(args, make_format("gray", 8, False), clip.width, clip.height, 3, 0)
+    src = torch_cuda.from_numpy(np.ascontiguousarray(clip.frames[0][0])).cuda()
+    out = torch_cuda.full((nb * n,), float("nan"), dtype=torch_cuda.float32, device="cuda")
+    torch_cuda.cuda.synchronize()
+    print(f"region_psd bw={bw} regions {rx}x{ry} blocks={nb}", flush=True)
+    e.region_psd(src.data_ptr(), src.stride(0), rx, ry, out.data_ptr())
+    e.sync()
+    got = out.cpu().numpy().astype(np.float64)
+    err = np.max(np.abs(got - want)) / np.max(want)
+    print(f"region_psd bw={bw} max rel err {err:.3g}", flush=True)
+    assert np.isfinite(got).all() and err <= 1e-4
+    if rx >= geo[0] and ry >= geo[1]:
+        assert np.array_equal(got.reshape(nb, n), np.broadcast_to(got[:n], (nb, n)))
+    e.close()
+
+
 @pytest.mark.parametrize("args", ["sigma=3;bt=3;bw=16;bh=16;pfactor=1;pregion=3",
                                   "sigma=2;bt=3;bw=48;bh=48;ow=16;oh=16;pfactor=1;pregion=2,1;sharpen=0.3"])
 def test_run_device_region_patterns(torch_cuda, oracle, args):
