@@ -219,32 +219,41 @@ def gpu_run(args):
 
 
 def host_fed(planes, args):
-    """Frames/s of the whole clip handed over in host memory (numpy planes)
-    and returned to host memory by nf3d_run_host_batch: the caller's planes
-    copied to pinned staging by host threads, uploaded, filtered and
-    downloaded chunk by chunk on three streams."""
+    """Frames/s of the whole clip handed over in host memory and returned to
+    host memory by nf3d_run_host_batch (PCIe-inclusive), from pageable numpy
+    planes (copied into pinned staging by host threads on the way) and from
+    page-locked planes (uploaded and returned by DMA directly): chunks
+    uploaded, filtered and downloaded on three streams."""
     import numpy as np
+    import torch
 
     from neo_fft3d_amd import Engine, default_params, make_format, run_host_batch
     F = args.frames
     fmt = make_format("yuv", 8)
     prm = default_params(**FILTER, bt=args.bt)
-    engines = [Engine(prm, fmt, W, H, F, p) for p in range(3)]
+
+    def rate(src_ptrs, src_strides, dst_ptrs, dst_strides):
+        engines = [Engine(prm, fmt, W, H, F, p) for p in range(3)]
+        items = [(engines[p], 0, F, src_ptrs[p], src_strides[p], dst_ptrs[p], dst_strides[p]) for p in range(3)]
+        run_host_batch(items)  # warm-up (device slots and staging, first transforms)
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            run_host_batch(items)
+        dt = time.perf_counter() - t0
+        for e in engines:
+            e.close()
+        return round(F * reps / dt, 1)
     outs = [np.empty_like(x) for x in planes]
-    items = [(engines[p], 0, F, [planes[p][n].ctypes.data for n in range(F)], planes[p].strides[1],
-              [outs[p][n].ctypes.data for n in range(F)], outs[p].strides[1]) for p in range(3)]
-    run_host_batch(items)  # warm-up (allocations, first transforms)
-    reps = 3
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        run_host_batch(items)
-    dt = time.perf_counter() - t0
-    for e in engines:
-        e.close()
+    pageable = rate([[x[n].ctypes.data for n in range(F)] for x in planes], [x.strides[1] for x in planes],
+                    [[o[n].ctypes.data for n in range(F)] for o in outs], [o.strides[1] for o in outs])
+    pin_in = [torch.from_numpy(x).pin_memory() for x in planes]
+    pin_out = [torch.empty_like(x).pin_memory() for x in pin_in]
+    pinned = rate([[x[n].data_ptr() for n in range(F)] for x in pin_in], [x.stride(1) for x in pin_in],
+                  [[o[n].data_ptr() for n in range(F)] for o in pin_out], [o.stride(1) for o in pin_out])
     mb = sum(x[0].nbytes for x in planes) / 1e6
-    return {"fps": round(F * reps / dt, 1), "unit": "frames/s",
-            "note": f"PCIe-inclusive: {mb:.2f} MB in and out per frame from pageable host memory; "
-                    f"{F} frames x {reps} passes"}
+    return {"fps_pageable": pageable, "fps_pinned": pinned, "unit": "frames/s",
+            "note": f"PCIe-inclusive: {mb:.2f} MB in and out per frame; {F} frames x 3 passes per form"}
 
 
 def precision_form(args):
@@ -299,11 +308,12 @@ def precision_form(args):
 
 
 def precision_filter(args):
-    """The filter itself in the precision form (dft=1: its 2D transforms as
-    matrix-core DFT products, in the frame loop) on YUV420P10 3840x2160,
-    bt=3, 32/16: frames/s of both forms on a resident clip and how often their
-    outputs agree.
-    (The error of each form against FP64 is measured by the tests.)"""
+    """The precision workload's matrix on YUV420P10 3840x2160 (sigma 3 scaled
+    to 12 in 10-bit units, noise 12 / 8): bt = 0 (Kalman, kratio 2) and bt = 3,
+    each at 32/16 and 64/16, all planes, frames/s of the FFT form on a
+    resident clip; at 32/16 also the DFT-as-GEMM form (dft=1, matrix cores)
+    and how often the two forms' outputs agree.  (Each form's per-plane error
+    against FP64 is measured by the tests.)"""
     import numpy as np
     import torch
 
@@ -312,12 +322,15 @@ def precision_filter(args):
     L = lib()
     L.nf3d_engine_set_dft.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
     F, w, h = args.precision_frames, 3840, 2160
-    planes = rects_clip(w, h, F, SEED_BASE + 3, bits=10, noise_luma=3.0, noise_chroma=3.0)
+    planes = rects_clip(w, h, F, SEED_BASE + 3, bits=10, noise_luma=12.0, noise_chroma=8.0)
     src = [torch.from_numpy(p).cuda() for p in planes]
     fmt = make_format("yuv", 10)
-    out, fps = {}, {}
-    for dft in (0, 1):
-        es = [Engine(default_params(3.0, bt=3, bw=32, bh=32, ow=16, oh=16), fmt, w, h, F, p) for p in range(3)]
+
+    def run(bt, bw, dft):
+        kw = dict(bt=bt, bw=bw, bh=bw, ow=16, oh=16)
+        if bt == 0:
+            kw["kratio"] = 2.0
+        es = [Engine(default_params(3.0, **kw), fmt, w, h, F, p) for p in range(3)]
         if dft:
             for e in es:
                 assert L.nf3d_engine_set_dft(e.h, 1, None, 0) == 0
@@ -325,23 +338,31 @@ def precision_filter(args):
         items = [(e, 0, F, [src[p][k].data_ptr() for k in range(F)], src[p][0].stride(0) * 2,
                   [dst[p][k].data_ptr() for k in range(F)], src[p][0].stride(0) * 2) for p, e in enumerate(es)]
         torch.cuda.synchronize()
+        run_device_batch(items)  # warm-up pass (the Kalman state then carries on: same work per pass)
+        for e in es:
+            e.sync()
+        out = [d.cpu().numpy() for d in dst]
+        t = time.perf_counter()
         run_device_batch(items)
         for e in es:
             e.sync()
-        out[dft] = [d.cpu().numpy() for d in dst]
-        t = time.perf_counter()
-        for _ in range(2):
-            run_device_batch(items)
-        for e in es:
-            e.sync()
-        fps[dft] = 2 * F / (time.perf_counter() - t)
+        fps = F / (time.perf_counter() - t)
         for e in es:
             e.close()
-    d = [np.abs(a.astype(np.int32) - b.astype(np.int32)) for a, b in zip(out[0], out[1])]
-    return {"workload": "yuv420p10_3840x2160_bt3_bw32_ow16", "frames": F,
-            "fps_fft_frame_loop": round(fps[0], 1), "fps_mfma_dft_frame_loop": round(fps[1], 1),
-            "forms_max_diff_lsb": int(max(x.max() for x in d)),
-            "forms_equal_frac": round(float(np.mean([np.mean(x == 0) for x in d])), 6)}
+        return fps, out
+    res = []
+    for bt in (0, 3):
+        for bw in (32, 64):
+            fps, out_fft = run(bt, bw, 0)
+            row = {"bt": bt, "bw": bw, "ow": 16, "fps_fft": round(fps, 1)}
+            if bw == 32:
+                fps_m, out_m = run(bt, bw, 1)
+                d = [np.abs(a.astype(np.int32) - b.astype(np.int32)) for a, b in zip(out_fft, out_m)]
+                row.update({"fps_mfma_dft": round(fps_m, 1), "mfma_over_fft": round(fps_m / fps, 3),
+                            "forms_max_diff_lsb": int(max(x.max() for x in d)),
+                            "forms_equal_frac": round(float(np.mean([np.mean(x == 0) for x in d])), 6)})
+            res.append(row)
+    return {"workload": "yuv420p10_3840x2160", "frames": F, "runs": res}
 
 
 def multi_clip(args):
@@ -788,7 +809,7 @@ def main():
                     help="skip the DFT-as-GEMM (MFMA, FP16 split) transform measurement")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-PSD leg")
     ap.add_argument("--learned-frames", type=int, default=96)
-    ap.add_argument("--precision-frames", type=int, default=8)
+    ap.add_argument("--precision-frames", type=int, default=120)
     ap.add_argument("--no-multi-clip", action="store_true", help="skip the 16-clip batched leg")
     ap.add_argument("--multi-frames", type=int, default=40)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",

@@ -30,6 +30,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/nf3d.h"
@@ -206,6 +207,15 @@ struct nf3d_engine {
   void* d_lx = nullptr;
   float* d_lpat = nullptr;
   void* d_lwork = nullptr;
+  // batched prediction for the engines of one frame-loop launch that share a
+  // model (this engine first among them): features and PSDs of all their
+  // blocks, rows in job order, and the model's workspace for that many rows
+  void* d_bx = nullptr;
+  float* d_bp = nullptr;
+  void* d_bwork = nullptr;
+  long long b_rows = 0;
+  long long b_work = 0;
+  std::vector<void*> b_retired;  // outgrown batch buffers (a captured graph may still use them)
   // per-region noise patterns (regions of preg_x x preg_y blocks; 0 = the
   // reference's single pattern): one pattern per block, 2D and 3D
   int preg_x = 0, preg_y = 0;
@@ -229,6 +239,10 @@ struct nf3d_engine {
     hipFree(d_lx);
     hipFree(d_lpat);
     hipFree(d_lwork);
+    hipFree(d_bx);
+    hipFree(d_bp);
+    hipFree(d_bwork);
+    for (void* q : b_retired) hipFree(q);
     hipFree(d_rpat2d);
     hipFree(d_rpat3d);
     hipFree(d_rspec);
@@ -1222,6 +1236,83 @@ namespace {
 // The batch: plans of every item, prologues, the frame-loop launches grouped
 // by (block shape, mode, bt, sharpen case, transform form) and launch index,
 // epilogues.
+// The learned-PSD predictions of one frame-loop launch: the PSD of each
+// job's first frame m0 (for Wiener 3D with bt >= 3 its spectrum is in ring
+// slot m0 % bt already -- frame m0 is an older frame of its own window -- so
+// the prediction starts from there instead of transforming the frame again).
+// Jobs that share a model and an output scale (the planes of a clip) are
+// predicted together: one feature pass per job into the first one's batch
+// buffer, then one set of GEMMs over all their blocks, each job's pattern
+// pointer set to its rows.
+void learned_predictions(const nf3d_run_item* items, const std::vector<nf3d_engine::RangePlan>& plans,
+                         const std::vector<int>& members, size_t c0, nf3d::RunBatch& B, hipStream_t s) {
+  std::vector<bool> done(B.njobs, false);
+  for (int c = 0; c < B.njobs; c++) {
+    const int k = members[c0 + c];
+    nf3d_engine* e = items[k].engine;
+    if (!e->learned || done[c]) continue;
+    const float mult = plans[k].mode == nf3d::MODE_W3D ? (float)e->ep.bt : 1.0f;
+    const float scale = mult * e->learned_strength;
+    std::vector<int> group;
+    for (int c2 = c; c2 < B.njobs; c2++) {
+      const int k2 = members[c0 + c2];
+      nf3d_engine* e2 = items[k2].engine;
+      const float mult2 = plans[k2].mode == nf3d::MODE_W3D ? (float)e2->ep.bt : 1.0f;
+      if (!done[c2] && e2->learned == e->learned && mult2 * e2->learned_strength == scale) {
+        group.push_back(c2);
+        done[c2] = true;
+      }
+    }
+    auto spectra = [&](int c2) -> const float2* {
+      const int k2 = members[c0 + c2];
+      nf3d_engine* e2 = items[k2].engine;
+      const RunLaunch& R = B.job[c2];
+      if (plans[k2].mode == nf3d::MODE_W3D && e2->ep.bt >= 3) return e2->slot_ptr(R.m0 % e2->ep.bt);
+      PlaneJob F = e2->base_job();
+      F.src = static_cast<const uint8_t*>(plans[k2].src[R.m0]);
+      F.src_pitch = plans[k2].src_pitch;
+      F.spec_out = e2->d_lspec;
+      e2->launch(nf3d::MODE_FWD, 0, F, s);
+      return e2->d_lspec;
+    };
+    if (group.size() == 1) {
+      e->learned_psd_spec(spectra(c), mult, s);
+      continue;
+    }
+    long long rows = 0;
+    for (int c2 : group) rows += items[members[c0 + c2]].engine->nb;
+    const long long n = (long long)e->ep.bh * e->outwidth;
+    const long long work = nf3d_mlp_work_bytes(e->learned, (int)rows);
+    if (rows > e->b_rows) {
+      if (e->d_bx) e->b_retired.push_back(e->d_bx);
+      if (e->d_bp) e->b_retired.push_back(e->d_bp);
+      e->d_bx = nullptr;
+      e->d_bp = nullptr;
+      e->d_bx = dev_alloc<uint16_t>((size_t)(rows * n));
+      e->d_bp = dev_alloc<float>((size_t)(rows * n));
+      e->b_rows = rows;
+    }
+    if (work > e->b_work) {
+      if (e->d_bwork) e->b_retired.push_back(e->d_bwork);
+      e->d_bwork = nullptr;
+      e->d_bwork = dev_alloc<uint8_t>((size_t)work);
+      e->b_work = work;
+    }
+    long long row = 0;
+    for (int c2 : group) {
+      nf3d_engine* e2 = items[members[c0 + c2]].engine;
+      if (nf3d_mlp_features(e->learned, spectra(c2), e2->nb, static_cast<uint16_t*>(e->d_bx) + row * n, s))
+        fail("learned PSD: feature kernel");
+      B.job[c2].J.pattern2d = e->d_bp + row * n;
+      B.job[c2].J.pattern3d = e->d_bp + row * n;
+      row += e2->nb;
+    }
+    char msg[256] = {0};
+    if (nf3d_mlp_predict(e->learned, e->d_bx, (int)rows, e->d_bp, 1, scale, e->d_bwork, s, msg, sizeof msg))
+      fail(std::string("learned PSD: ") + msg);
+  }
+}
+
 void run_batch(const nf3d_run_item* items, int n, hipStream_t s, bool timed, bool capture = false) {
   if (!capture)
     for (int i = 0; i < n; i++) items[i].engine->order_before(s);
@@ -1258,23 +1349,13 @@ void run_batch(const nf3d_run_item* items, int n, hipStream_t s, bool timed, boo
           nf3d_engine* e = items[k].engine;
           RunLaunch& R = B.job[B.njobs++];
           R = plans[k].launches[l];
-          if (e->learned) {
-            // the PSD of the launch's first frame m0; for Wiener 3D with
-            // bt >= 3 its spectrum is in ring slot m0 % bt already (frame m0
-            // is an older frame of its own window), so the prediction starts
-            // from there instead of transforming the frame again
-            if (plans[k].mode == nf3d::MODE_W3D && e->ep.bt >= 3)
-              e->learned_psd_spec(e->slot_ptr(R.m0 % e->ep.bt), (float)e->ep.bt, s);
-            else
-              e->learned_psd_plane(static_cast<const uint8_t*>(plans[k].src[R.m0]), plans[k].src_pitch,
-                                   plans[k].mode == nf3d::MODE_W3D ? (float)e->ep.bt : 1.0f, s);
-          }
           R.wg_begin = B.total_wg;
           B.total_wg += R.ntiles * ((R.nframes + R.fpw - 1) / R.fpw);
           B.max_hc = std::max(B.max_hc, e->Hc);
           B.max_frames = std::max(B.max_frames, R.nframes);
-          evs.push_back(e->timer(timed, R.nframes, s));
         }
+        learned_predictions(items, plans, members, c0, B, s);
+        for (int c = 0; c < B.njobs; c++) evs.push_back(items[members[c0 + c]].engine->timer(timed, B.job[c].nframes, s));
         // the seam pass follows on the same stream: run beside the next
         // launch it takes issue slots the latency-bound k_run needs more
         check(nf3d::launch_run(plans[i].mode, e0->ep.bt, plans[i].shp, e0->dft_mode, e0->ep.bw, e0->ep.bh, B, s),
@@ -1376,6 +1457,23 @@ struct CopyJob {
   size_t row;
   int rows;
 };
+// Whether p is page-locked host memory the DMA engines can read directly
+// (hipHostMalloc'd; pageable memory is not known to the runtime).
+bool is_pinned(const void* p) {
+  hipPointerAttribute_t a{};
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeHost;
+}
+
+// Device slots and pinned staging of an engine's host-fed calls, kept from
+// one call to the next and released with the engine (the map itself is never
+// destroyed: no HIP call from static destructors after the runtime is gone).
+std::mutex g_host_mu;
+auto& g_host_planes = *new std::unordered_map<const nf3d_engine*, std::unique_ptr<HostPlane>>;
+
 void run_copies(const std::vector<CopyJob>& jobs) {
   if (jobs.empty()) return;
   const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -1442,28 +1540,42 @@ int nf3d_run_host_batch(const nf3d_run_item* items, int n, char* err, size_t err
     for (auto& e : ev_up) e = new_event();
     for (auto& e : ev_run) e = new_event();
     for (auto& e : ev_down) e = new_event();
-    std::vector<std::unique_ptr<HostPlane>> pl;
+    std::vector<HostPlane*> pl;
+    std::vector<char> pin_src(n), pin_dst(n);  // the caller's planes are pinned: no staging copy
     for (int i = 0; i < n; i++) {
       nf3d_engine* e = items[i].engine;
-      auto P = std::make_unique<HostPlane>();
-      P->e = e;
-      P->row = (size_t)e->framewidth * e->pix;
-      P->rows = e->frameheight;
-      P->pitch = (P->row + 255) / 256 * 256;
-      const size_t slot = P->pitch * P->rows;
-      P->d_in = dev_alloc<uint8_t>(slot * kInSlots);
-      P->d_out = dev_alloc<uint8_t>(slot * 2 * kChunk);
-      for (int b = 0; b < 2; b++) {
-        check(hipHostMalloc((void**)&P->h_in[b], slot * (kChunk + 4), hipHostMallocDefault), "hipHostMalloc");
-        check(hipHostMalloc((void**)&P->h_out[b], slot * kChunk, hipHostMallocDefault), "hipHostMalloc");
+      std::unique_ptr<HostPlane>* cached;
+      {
+        std::lock_guard<std::mutex> lock(g_host_mu);
+        cached = &g_host_planes[e];
       }
-      P->dsrc.assign(F, nullptr);
-      P->ddst.assign(F, nullptr);
-      for (int k = 0; k < F; k++) {
-        P->dsrc[k] = P->d_in + (size_t)(k % kInSlots) * slot;
-        P->ddst[k] = P->d_out + (size_t)(k % (2 * kChunk)) * slot;
+      if (!*cached) {
+        auto P = std::make_unique<HostPlane>();
+        P->e = e;
+        P->row = (size_t)e->framewidth * e->pix;
+        P->rows = e->frameheight;
+        P->pitch = (P->row + 255) / 256 * 256;
+        const size_t slot = P->pitch * P->rows;
+        P->d_in = dev_alloc<uint8_t>(slot * kInSlots);
+        P->d_out = dev_alloc<uint8_t>(slot * 2 * kChunk);
+        for (int b = 0; b < 2; b++) {
+          check(hipHostMalloc((void**)&P->h_in[b], slot * (kChunk + 4), hipHostMallocDefault), "hipHostMalloc");
+          check(hipHostMalloc((void**)&P->h_out[b], slot * kChunk, hipHostMallocDefault), "hipHostMalloc");
+        }
+        P->dsrc.assign(F, nullptr);
+        P->ddst.assign(F, nullptr);
+        for (int k = 0; k < F; k++) {
+          P->dsrc[k] = P->d_in + (size_t)(k % kInSlots) * slot;
+          P->ddst[k] = P->d_out + (size_t)(k % (2 * kChunk)) * slot;
+        }
+        *cached = std::move(P);
       }
-      pl.push_back(std::move(P));
+      pl.push_back(cached->get());
+      bool ps = true, pd = true;
+      for (int k = std::max(0, first - 2); k < std::min(F, first + count + 2) && ps; k++) ps = is_pinned(items[i].src[k]);
+      for (int k = first; k < first + count && pd; k++) pd = is_pinned(items[i].dst[k]);
+      pin_src[i] = ps;
+      pin_dst[i] = pd;
     }
     for (int i = 0; i < n; i++) items[i].engine->order_before(s_up);
     // frames a chunk's filter reads: its own and the window's on either side
@@ -1489,6 +1601,7 @@ int nf3d_run_host_batch(const nf3d_run_item* items, int n, char* err, size_t err
       std::vector<CopyJob> jobs;
       for (int i = 0; i < n; i++) {
         HostPlane& P = *pl[i];
+        if (pin_src[i]) continue;
         for (int k = lo; k <= hi; k++)
           jobs.push_back(CopyJob{P.h_in[buf] + (size_t)(k - lo) * P.pitch * P.rows, P.pitch,
                                  static_cast<const uint8_t*>(items[i].src[k]), (size_t)items[i].src_stride, P.row,
@@ -1499,10 +1612,16 @@ int nf3d_run_host_batch(const nf3d_run_item* items, int n, char* err, size_t err
       if (c >= 2) check(hipStreamWaitEvent(s_up, ev_run[c - 2], 0), "hipStreamWaitEvent");
       for (int i = 0; i < n; i++) {
         HostPlane& P = *pl[i];
-        for (int k = lo; k <= hi; k++)
-          check(hipMemcpyAsync(const_cast<void*>(P.dsrc[k]), P.h_in[buf] + (size_t)(k - lo) * P.pitch * P.rows,
-                               P.pitch * P.rows, hipMemcpyHostToDevice, s_up),
-                "upload");
+        for (int k = lo; k <= hi; k++) {
+          if (pin_src[i])
+            check(hipMemcpy2DAsync(const_cast<void*>(P.dsrc[k]), P.pitch, items[i].src[k], (size_t)items[i].src_stride,
+                                   P.row, P.rows, hipMemcpyHostToDevice, s_up),
+                  "upload");
+          else
+            check(hipMemcpyAsync(const_cast<void*>(P.dsrc[k]), P.h_in[buf] + (size_t)(k - lo) * P.pitch * P.rows,
+                                 P.pitch * P.rows, hipMemcpyHostToDevice, s_up),
+                  "upload");
+        }
       }
       check(hipEventRecord(ev_up[c], s_up), "hipEventRecord");
     };
@@ -1513,6 +1632,7 @@ int nf3d_run_host_batch(const nf3d_run_item* items, int n, char* err, size_t err
       for (int i = 0; i < n; i++) {
         HostPlane& P = *pl[i];
         nf3d_engine* e = P.e;
+        if (pin_dst[i]) continue;  // downloaded straight into the caller's planes
         // the processed region only: crop borders stay the caller's
         for (int k = a; k <= b; k++)
           jobs.push_back(CopyJob{static_cast<uint8_t*>(items[i].dst[k]) + (size_t)e->tc * items[i].dst_stride +
@@ -1540,10 +1660,21 @@ int nf3d_run_host_batch(const nf3d_run_item* items, int n, char* err, size_t err
       check(hipStreamWaitEvent(s_down, ev_run[c], 0), "hipStreamWaitEvent");
       for (int i = 0; i < n; i++) {
         HostPlane& P = *pl[i];
-        for (int k = a; k < a + cnt; k++)
-          check(hipMemcpyAsync(P.h_out[c & 1] + (size_t)(k - a) * P.pitch * P.rows, P.ddst[k], P.pitch * P.rows,
-                               hipMemcpyDeviceToHost, s_down),
-                "download");
+        nf3d_engine* e = P.e;
+        for (int k = a; k < a + cnt; k++) {
+          if (pin_dst[i])
+            check(hipMemcpy2DAsync(static_cast<uint8_t*>(items[i].dst[k]) + (size_t)e->tc * items[i].dst_stride +
+                                       (size_t)e->lc * e->pix,
+                                   (size_t)items[i].dst_stride,
+                                   static_cast<const uint8_t*>(P.ddst[k]) + (size_t)e->tc * P.pitch +
+                                       (size_t)e->lc * e->pix,
+                                   P.pitch, (size_t)e->Wc * e->pix, e->Hc, hipMemcpyDeviceToHost, s_down),
+                  "download");
+          else
+            check(hipMemcpyAsync(P.h_out[c & 1] + (size_t)(k - a) * P.pitch * P.rows, P.ddst[k], P.pitch * P.rows,
+                                 hipMemcpyDeviceToHost, s_down),
+                  "download");
+        }
       }
       check(hipEventRecord(ev_down[c], s_down), "hipEventRecord");
       // meanwhile: the next chunk's frames up, the previous chunk's out
@@ -1912,6 +2043,12 @@ int nf3d_engine_sync(nf3d_engine* e) {
   return 0;
 }
 
-void nf3d_engine_destroy(nf3d_engine* e) { delete e; }
+void nf3d_engine_destroy(nf3d_engine* e) {
+  {
+    std::lock_guard<std::mutex> lock(g_host_mu);
+    g_host_planes.erase(e);
+  }
+  delete e;
+}
 
 }  // extern "C"

@@ -313,10 +313,13 @@ __device__ __forceinline__ void gather_cols(const PlaneJob& J, float2* stage, co
   if (b >= nblk) return;  // lanes past the tile's blocks
   const int sx = reflect101((bx0 + b) * stepx + j - J.mirw, J.Wc) + J.lc;
   const float wx = winx[j], base = J.base;
+  // 32-bit byte offsets from the uniform plane base: one add per sample
+  // (rows are < 2^31 bytes from the base: checked by the host)
+  const int sxb = sx * (int)sizeof(T);
   T v[BH];
   static_for<0, BH>([&](auto ii) {
     constexpr int i = decltype(ii)::value;
-    v[i] = reinterpret_cast<const T*>(src + rowoff[i])[sx];
+    v[i] = *reinterpret_cast<const T*>(src + (unsigned)(rowoff[i] + sxb));
   });
   float* d = reinterpret_cast<float*>(&stage[b * LA + j]);
   static_for<0, BH>([&](auto ii) {
@@ -438,7 +441,7 @@ __device__ __forceinline__ float2 apply2d_s(const SpecParams& J, float2 x, const
   const float cr = x.x - g0, ci = x.y - g1;
   const float psd = power(cr, ci);
   float f = fmaf_(-T.sig, __builtin_amdgcn_rcpf(psd), 1.0f);
-  f = f > J.lowlimit ? f : J.lowlimit;
+  f = __builtin_fmaxf(f, J.lowlimit);  // f is never NaN (psd >= 1e-15)
   if constexpr (SHP != SHP_NONE) {
     if (!J.use_pattern) f *= shp_factor<SHP>(J, psd, T.wsh, T.wdh);
   }
@@ -570,9 +573,11 @@ __global__ void __launch_bounds__(NT, (run_blocks_per_cu<BH, NT, MODE, BT, SHP>(
   using TBuf = TabBuf<SHP, CH>;
   __shared__ __align__(16) unsigned char smem[S::BYTES];
   __shared__ float s_gf[BPW], s_gfs[BPW];
-  __shared__ float winx[BW], winy[BH];
-  __shared__ float wsyn[2 * BW + 2 * BH];  // synthesis windows: left, right, top, bottom bands
-  __shared__ int rowoff[BH];  // byte offset of the source row of each tile row (< 2^31: checked by the host)
+  __shared__ __align__(16) float winx[BW];
+  __shared__ __align__(16) float winy[BH];
+  __shared__ __align__(16) float wsyn[2 * BW + 2 * BH];  // synthesis windows: left, right, top, bottom bands
+  // byte offset of the source row of each tile row (< 2^31: checked by the host)
+  __shared__ __align__(16) int rowoff[BH];
   float2* spec = reinterpret_cast<float2*>(smem);
   float2* stage = reinterpret_cast<float2*>(smem);
   float* syn = reinterpret_cast<float*>(smem);

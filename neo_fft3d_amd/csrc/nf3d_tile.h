@@ -134,7 +134,7 @@ __device__ __forceinline__ float dehalo_factor(const PJ& J, float psd, float wdh
 // max(1 - sig / psd, lowlimit) (src/code_impl/Apply.cpp:13-235).
 __device__ __forceinline__ float wiener_gain(float re, float im, float sig, float lowlimit) {
   const float f = fmaf_(-sig, __builtin_amdgcn_rcpf(power(re, im)), 1.0f);
-  return f > lowlimit ? f : lowlimit;
+  return __builtin_fmaxf(f, lowlimit);  // f is never NaN (psd >= 1e-15)
 }
 
 // Apply2D (src/code_impl/Apply.cpp:13-65): one element.
@@ -147,7 +147,7 @@ __device__ __forceinline__ float2 apply2d_elem(const PlaneJob& J, float2 x, cons
   float cr = x.x - g0, ci = x.y - g1;
   float psd = power(cr, ci);
   float f = fmaf_(-T.sig, __builtin_amdgcn_rcpf(psd), 1.0f);
-  f = f > J.lowlimit ? f : J.lowlimit;
+  f = __builtin_fmaxf(f, J.lowlimit);
   if (!J.use_pattern) {
     if (J.dehalo == 0.0f) f *= sharpen_factor(J, psd, T.wsh);
     else if (J.sharpen == 0.0f) f *= dehalo_factor(J, psd, T.wdh);
