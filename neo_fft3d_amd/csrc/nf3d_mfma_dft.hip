@@ -15,7 +15,7 @@
 namespace nf3d {
 
 // Forward: in [nb][32][32] float -> out [nb][32][17] float2 (FFTW r2c layout).
-__global__ void __launch_bounds__(256) k_dft32_fwd(const float* in, float2* out, int nb, float sx, const H2* T) {
+__global__ void __launch_bounds__(256) k_dft32_fwd(const float* in, float2* out, int nb, float sx, const H8* T) {
   const int lane = threadIdx.x & 63;
   const int blk = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (blk >= nb) return;  // whole wavefronts
@@ -26,7 +26,7 @@ __global__ void __launch_bounds__(256) k_dft32_fwd(const float* in, float2* out,
 }
 
 // Inverse: in [nb][32][17] float2 -> out [nb][32][32] float (FFTW c2r, unnormalised).
-__global__ void __launch_bounds__(256) k_dft32_inv(const float2* in, float* out, int nb, float sz, const H2* T) {
+__global__ void __launch_bounds__(256) k_dft32_inv(const float2* in, float* out, int nb, float sz, const H8* T) {
   const int lane = threadIdx.x & 63;
   const int blk = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (blk >= nb) return;
@@ -41,9 +41,9 @@ __global__ void __launch_bounds__(256) k_dft32_inv(const float2* in, float* out,
 // ----------------------------------------------------------------- tables
 namespace {
 
-void put(std::vector<H2>& T, int t, int lane, const double (&v)[4]) {
-  H2 h;
-  for (int i = 0; i < 4; i++) {
+void put(std::vector<H8>& T, int t, int lane, const double (&v)[8]) {
+  H8 h;
+  for (int i = 0; i < 8; i++) {
     const float x = (float)v[i];
     const _Float16 hi = (_Float16)x;
     h.hi[i] = hi;
@@ -52,29 +52,34 @@ void put(std::vector<H2>& T, int t, int lane, const double (&v)[4]) {
   T[(size_t)t * 64 + lane] = h;
 }
 
-// A tile (m, ks) of a 32x32 matrix M(row, col): lane l -> M[16m + l%16][16ks + 4(l/16) + i]
-template <class F>
-void a_tile(std::vector<H2>& T, int t, int m, int ks, F M) {
+// K index of value i of lane group g: memory order, or the order in which a
+// lane holds the rows of the previous product's two D tiles (see nf3d_mfma.h)
+int k_mem(int g, int i) { return 8 * g + i; }
+int k_dtile(int g, int i) { return i < 4 ? 4 * g + i : 16 + 4 * g + (i - 4); }
+
+// A tile (row tile m) of a 32x32 matrix M(row, col): lane l -> M[16m + l%16][K(l/16, i)]
+template <class F, class K>
+void a_tile(std::vector<H8>& T, int t, int m, F M, K kidx) {
   for (int l = 0; l < 64; l++) {
-    double v[4];
-    for (int i = 0; i < 4; i++) v[i] = M(16 * m + l % 16, 16 * ks + 4 * (l / 16) + i);
+    double v[8];
+    for (int i = 0; i < 8; i++) v[i] = M(16 * m + l % 16, kidx(l / 16, i));
     put(T, t, l, v);
   }
 }
 
-// B tile (ks, n): lane l -> M[16ks + 4(l/16) + i][16n + l%16]
+// B tile (column tile n): lane l -> M[K(l/16, i)][16n + l%16], memory K order
 template <class F>
-void b_tile(std::vector<H2>& T, int t, int ks, int n, F M) {
+void b_tile(std::vector<H8>& T, int t, int n, F M) {
   for (int l = 0; l < 64; l++) {
-    double v[4];
-    for (int i = 0; i < 4; i++) v[i] = M(16 * ks + 4 * (l / 16) + i, 16 * n + l % 16);
+    double v[8];
+    for (int i = 0; i < 8; i++) v[i] = M(k_mem(l / 16, i), 16 * n + l % 16);
     put(T, t, l, v);
   }
 }
 
-std::vector<H2> build_tables() {
+std::vector<H8> build_tables() {
   const double w = 2.0 * 3.14159265358979323846 / 32.0;
-  std::vector<H2> T((size_t)DftTab::COUNT * 64);
+  std::vector<H8> T((size_t)DftTab::COUNT * 64);
   auto C = [&](int j, int c) {
     if (c < 17) return std::cos(w * ((j * c) % 32));
     if (c >= 32 && c < 49) return -std::sin(w * ((j * (c - 32)) % 32));
@@ -84,29 +89,27 @@ std::vector<H2> build_tables() {
   auto Fi = [&](int a, int b) { return -std::sin(w * ((a * b) % 32)); };
   auto Fn = [&](int a, int b) { return std::sin(w * ((a * b) % 32)); };
   auto wk = [](int k) { return (k == 0 || k == 16) ? 1.0 : 2.0; };
-  auto Dr = [&](int j, int k) { return k < 17 ? wk(k) * std::cos(w * ((j * k) % 32)) : 0.0; };
-  auto Dn = [&](int j, int k) { return k < 17 ? -wk(k) * std::sin(w * ((j * k) % 32)) : 0.0; };
-  for (int ks = 0; ks < 2; ks++)
-    for (int n = 0; n < 4; n++) b_tile(T, DftTab::C + ks * 4 + n, ks, n, C);
-  for (int m = 0; m < 2; m++)
-    for (int ks = 0; ks < 2; ks++) {
-      a_tile(T, DftTab::FR + m * 2 + ks, m, ks, Fr);
-      a_tile(T, DftTab::FI + m * 2 + ks, m, ks, Fi);
-      a_tile(T, DftTab::DR + m * 2 + ks, m, ks, Dr);
-      a_tile(T, DftTab::DN + m * 2 + ks, m, ks, Dn);
-    }
-  for (int ks = 0; ks < 2; ks++)
-    for (int n = 0; n < 2; n++) {
-      b_tile(T, DftTab::BR + ks * 2 + n, ks, n, Fr);
-      b_tile(T, DftTab::BI + ks * 2 + n, ks, n, Fi);
-      b_tile(T, DftTab::BN + ks * 2 + n, ks, n, Fn);
-    }
+  // Dr^T / Dn^T: row j (output column of x), column k (spectral column)
+  auto DrT = [&](int j, int k) { return k < 17 ? wk(k) * std::cos(w * ((j * k) % 32)) : 0.0; };
+  auto DnT = [&](int j, int k) { return k < 17 ? -wk(k) * std::sin(w * ((j * k) % 32)) : 0.0; };
+  for (int n = 0; n < 4; n++) b_tile(T, DftTab::C + n, n, C);
+  for (int m = 0; m < 2; m++) {
+    a_tile(T, DftTab::FR + m, m, Fr, k_dtile);
+    a_tile(T, DftTab::FI + m, m, Fi, k_dtile);
+    a_tile(T, DftTab::DR + m, m, DrT, k_dtile);
+    a_tile(T, DftTab::DN + m, m, DnT, k_dtile);
+  }
+  for (int n = 0; n < 2; n++) {
+    b_tile(T, DftTab::BR + n, n, Fr);
+    b_tile(T, DftTab::BI + n, n, Fi);
+    b_tile(T, DftTab::BN + n, n, Fn);
+  }
   return T;
 }
 
 }  // namespace
 
-const H2* mfma_dft_table(hipError_t* err) {
+const H8* mfma_dft_table(hipError_t* err) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) {
@@ -115,14 +118,14 @@ const H2* mfma_dft_table(hipError_t* err) {
   }
   // one copy of the tables per device, built on first use
   static std::mutex mu;
-  static std::vector<H2*> tabs;
+  static std::vector<H8*> tabs;
   std::lock_guard<std::mutex> lock(mu);
   if ((int)tabs.size() <= dev) tabs.resize(dev + 1, nullptr);
   if (!tabs[dev]) {
-    std::vector<H2> h = build_tables();
-    H2* d = nullptr;
-    e = hipMalloc(&d, h.size() * sizeof(H2));
-    if (e == hipSuccess) e = hipMemcpy(d, h.data(), h.size() * sizeof(H2), hipMemcpyHostToDevice);
+    std::vector<H8> h = build_tables();
+    H8* d = nullptr;
+    e = hipMalloc(&d, h.size() * sizeof(H8));
+    if (e == hipSuccess) e = hipMemcpy(d, h.data(), h.size() * sizeof(H8), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
       hipFree(d);
       (void)hipGetLastError();  // not left behind for a later launch check
@@ -140,7 +143,7 @@ hipError_t launch_dft_mfma(const void* in, void* out, int nb, int n, int inverse
   if (!(scale > 0.0f) || std::ldexp(1.0f, std::ilogb(scale)) != scale) return hipErrorInvalidValue;
   if (nb == 0) return hipSuccess;
   hipError_t e;
-  const H2* T = mfma_dft_table(&e);
+  const H8* T = mfma_dft_table(&e);
   if (!T) return e;
   dim3 grid((nb + 3) / 4), block(256);
   if (inverse)

@@ -528,7 +528,7 @@ __global__ void __launch_bounds__(NT) k_tile(TileLaunch L_arg) {
       static_assert(LSPEC >= LA && 2 * LA - LSPEC >= (BPW - 1) * (LSPEC - LA), "spectra overlap stages b, b+1 only");
       constexpr int NW = NT / 64, MB = (BPW + NW - 1) / NW;
       const int wave = __builtin_amdgcn_readfirstlane(t >> 6), lane = t & 63;
-      const H2* T = static_cast<const H2*>(J.dft_tab);
+      const DftTile* T = static_cast<const DftTile*>(J.dft_tab);
       const float sx = J.dft_sx;
 #pragma unroll 1
       for (int q = 0; q < MB; q++) {
@@ -694,7 +694,7 @@ __global__ void __launch_bounds__(NT) k_tile(TileLaunch L_arg) {
                     "synthesised rows overlap spectra b-1, b only");
       constexpr int NW = NT / 64, MB = (BPW + NW - 1) / NW;
       const int wave = __builtin_amdgcn_readfirstlane(t >> 6), lane = t & 63;
-      const H2* T = static_cast<const H2*>(J.dft_tab);
+      const DftTile* T = static_cast<const DftTile*>(J.dft_tab);
       const float sz = J.dft_sz;
 #pragma unroll 1
       for (int q = 0; q < MB; q++) {
