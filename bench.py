@@ -310,9 +310,10 @@ def precision_form(args):
 def precision_filter(args):
     """The precision workload's matrix on YUV420P10 3840x2160 (sigma 3 scaled
     to 12 in 10-bit units, noise 12 / 8): bt = 0 (Kalman, kratio 2) and bt = 3,
-    each at 32/16 and 64/16, all planes, frames/s of the FFT form on a
-    resident clip; at 32/16 also the DFT-as-GEMM form (dft=1, matrix cores)
-    and how often the two forms' outputs agree.  (Each form's per-plane error
+    each at 32/16 and 64/16, all planes: frames/s of the FFT form and of the
+    DFT-as-GEMM form (dft=1, matrix cores; the frame loop at 32x32, the
+    per-frame kernel at 64x64) on a resident clip, and how often the two
+    forms' outputs agree.  (Each form's per-plane error
     against FP64 is measured by the tests.)"""
     import numpy as np
     import torch
@@ -355,12 +356,12 @@ def precision_filter(args):
         for bw in (32, 64):
             fps, out_fft = run(bt, bw, 0)
             row = {"bt": bt, "bw": bw, "ow": 16, "fps_fft": round(fps, 1)}
-            if bw == 32:
-                fps_m, out_m = run(bt, bw, 1)
-                d = [np.abs(a.astype(np.int32) - b.astype(np.int32)) for a, b in zip(out_fft, out_m)]
-                row.update({"fps_mfma_dft": round(fps_m, 1), "mfma_over_fft": round(fps_m / fps, 3),
-                            "forms_max_diff_lsb": int(max(x.max() for x in d)),
-                            "forms_equal_frac": round(float(np.mean([np.mean(x == 0) for x in d])), 6)})
+            fps_m, out_m = run(bt, bw, 1)
+            d = [np.abs(a.astype(np.int32) - b.astype(np.int32)) for a, b in zip(out_fft, out_m)]
+            row.update({"fps_mfma_dft": round(fps_m, 1), "mfma_over_fft": round(fps_m / fps, 3),
+                        "mfma_kernel": "frame loop" if bw == 32 else "per-frame",
+                        "forms_max_diff_lsb": int(max(x.max() for x in d)),
+                        "forms_equal_frac": round(float(np.mean([np.mean(x == 0) for x in d])), 6)})
             res.append(row)
     return {"workload": "yuv420p10_3840x2160", "frames": F, "runs": res}
 

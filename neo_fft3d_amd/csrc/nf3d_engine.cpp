@@ -195,7 +195,8 @@ struct nf3d_engine {
   int run_fpw = 1;              // frames per workgroup, Wiener 2D / sharpen (tuning)
   bool run_frames_from_table = false;  // run_frames came from the tuning table, not the caller
   // 2D transforms: 0 = FFT in registers, 1 = DFT-matrix products on the matrix
-  // cores (32x32 blocks; in both the frame-loop and the per-frame kernel)
+  // cores (32x32 blocks in the frame-loop and the per-frame kernel, 64x64 in the
+  // per-frame kernel)
   int dft_mode = 0;
   const void* dft_tab = nullptr;
   float dft_sx = 1.0f, dft_sz = 1.0f;
@@ -1032,7 +1033,9 @@ struct nf3d_engine {
                       (long long)(tc + Hc) * dst_pitch < (1LL << 31) && 4LL * Hc * Wc < (1LL << 31) &&
                       dst_pitch % pix == 0;
     const bool pshow = ep.pfactor != 0 && ep.pshow;
-    P.run = P.a <= P.b && fits && !pshow && !(P.mode == nf3d::MODE_SHARPEN && P.shp == nf3d::SHP_NONE);
+    // the matrix-core form of 64x64 blocks runs in the per-frame kernel only
+    const bool mf64 = dft_mode && ep.bw == 64;
+    P.run = P.a <= P.b && fits && !pshow && !mf64 && !(P.mode == nf3d::MODE_SHARPEN && P.shp == nf3d::SHP_NONE);
     if (!P.run) return P;
     // launches of equal length (within one frame), two halves of seam
     // buffers in turn
@@ -1931,22 +1934,23 @@ int nf3d_engine_set_dft(nf3d_engine* e, int mode, char* err, size_t errlen) {
   try {
     if (mode != 0 && mode != 1) fail("dft mode must be 0 (FFT) or 1 (matrix-core DFT)");
     if (mode == 1 && !nf3d::mfma_dft_supported(e->ep.bw, e->ep.bh))
-      fail("the matrix-core DFT supports 32x32 blocks");
+      fail("the matrix-core DFT supports 32x32 and 64x64 blocks");
     // operands are scaled to FP16 from the format's range, which float
     // samples do not bound
     if (mode == 1 && e->fmt.is_float) fail("the matrix-core DFT supports integer formats only");
     check(hipSetDevice(e->device), "hipSetDevice");
     if (mode == 1 && !e->dft_tab) {
       hipError_t he;
-      e->dft_tab = nf3d::mfma_dft_table(&he);
+      e->dft_tab = nf3d::mfma_dft_table(e->ep.bw, &he);
       check(he, "matrix-core DFT tables");
     }
     // power-of-two operand scales keeping every FP16 operand below 2^14:
     // windowed samples up to the format's range (float: 4), spectra up to
     // bw*bh times that with headroom 4 for the gains
     const double maxabs = e->fmt.is_float ? 4.0 : (double)((1 << e->fmt.bits_per_sample) - 1);
+    const double area = (double)e->ep.bw * e->ep.bh;
     e->dft_sx = std::ldexp(1.0f, (int)std::floor(std::log2(16384.0 / maxabs)));
-    e->dft_sz = std::ldexp(1.0f, (int)std::floor(std::log2(16384.0 / (1024.0 * maxabs * 4.0))));
+    e->dft_sz = std::ldexp(1.0f, (int)std::floor(std::log2(16384.0 / (area * maxabs * 4.0))));
     e->dft_mode = mode;
   } catch (const Failure& f) {
     copy_err(err, errlen, f.msg);

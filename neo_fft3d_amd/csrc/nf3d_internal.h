@@ -138,7 +138,7 @@ hipError_t launch_pattern(const PatternJob& job, hipStream_t s);
 // every block gets its rx x ry region's pattern, 2D and times mult3d.
 hipError_t launch_pattern_regions(const PatternJob& job, int rx, int ry, float* out2d, float* out3d, float mult3d,
                                   hipStream_t s);
-// DFT-as-GEMM precision form (nf3d_mfma_dft.hip): n x n blocks, n = 32.
+// DFT-as-GEMM precision form (nf3d_mfma_dft.hip): n x n blocks, n = 32 or 64.
 hipError_t launch_dft_mfma(const void* in, void* out, int nb, int n, int inverse, float scale, hipStream_t s);
 // k_run over the batch's jobs; `shp` is a Shp value.  launch_seam_run
 // finishes the seams of their frames (after the k_run, same stream).

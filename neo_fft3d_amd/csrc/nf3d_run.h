@@ -695,7 +695,7 @@ __global__ void __launch_bounds__(NT, (run_blocks_per_cu<BH, NT, MODE, BT, SHP>(
       // gather stage of blocks b and b+1 (static_assert below), so a round
       // writes over stages its own round or earlier rounds have read, and one
       // barrier per round (operands loaded by both wavefronts) is enough.
-      static_assert(BW == 32 && BH == 32, "matrix-core DFT: 32x32 blocks");
+      static_assert(BW == BH && (BW == 32 || BW == 64), "matrix-core DFT: 32x32 or 64x64 blocks");
       static_assert(LSPEC >= LA && 2 * LA - LSPEC >= (BPW - 1) * (LSPEC - LA), "spectra overlap stages b, b+1 only");
       constexpr int NW = NT / 64, MB = (BPW + NW - 1) / NW;
       const int wave = wave_of(t), lane = t & 63;
@@ -704,9 +704,9 @@ __global__ void __launch_bounds__(NT, (run_blocks_per_cu<BH, NT, MODE, BT, SHP>(
 #pragma unroll 1
       for (int q = 0; q < MB; q++) {
         const int b = nblk - 1 - (q * NW + wave);  // uniform in the wavefront
-        DftIn din;
+        DftIn<BW> din;
         if (b >= 0)
-          din = dft32_fwd_load(
+          din = dft_fwd_load<BW>(
               [&](int r, int c) {
                 const float2 v = stage[b * LA + (r >> 1) * SA + c];
                 return (r & 1) ? v.y : v.x;
@@ -714,7 +714,7 @@ __global__ void __launch_bounds__(NT, (run_blocks_per_cu<BH, NT, MODE, BT, SHP>(
               sx, lane);
         __syncthreads();  // the round's stages read
         if (b >= 0)
-          dft32_fwd_compute(din, sx, T, lane, [&](int r, int k, float2 zz) { spec[b * LSPEC + r * SR + k] = zz; });
+          dft_fwd_compute<BW>(din, sx, T, lane, [&](int r, int k, float2 zz) { spec[b * LSPEC + r * SR + k] = zz; });
       }
     } else if (!row_wave) {
       __syncthreads();  // gather stage dead; spectral layout from here
@@ -959,11 +959,11 @@ __global__ void __launch_bounds__(NT, (run_blocks_per_cu<BH, NT, MODE, BT, SHP>(
 #pragma unroll 1
       for (int q = 0; q < MB; q++) {
         const int b = q * NW + wave;  // uniform in the wavefront
-        DftSpec dsp;
-        if (b < nblk) dsp = dft32_inv_load([&](int r, int k) { return spec[b * LSPEC + r * SR + k]; }, sz, lane);
+        DftSpec<BW> dsp;
+        if (b < nblk) dsp = dft_inv_load<BW>([&](int r, int k) { return spec[b * LSPEC + r * SR + k]; }, sz, lane);
         __syncthreads();  // the round's spectra read
         if (b < nblk)
-          dft32_inv_compute(dsp, sz, T, lane, [&](int r, int c0, f4 x) {
+          dft_inv_compute<BW>(dsp, sz, T, lane, [&](int r, int c0, f4 x) {
             float* d = syn + b * LSYN + r * SYNW + c0;
             d[0] = x[0];
             d[1] = x[1];
@@ -1205,6 +1205,8 @@ hipError_t launch_run_shape(int mode, int bt, int shp, int mf, const RunBatch& L
   }
   if (wg != L.total_wg) return hipErrorInvalidValue;
   if (mf) {
+    // 64x64 blocks in the matrix-core form go through the per-frame kernel
+    // (in the frame loop their operands would exceed the register file)
     if constexpr (BW == 32 && BH == 32) return launch_run_modes<BW, BH, true>(mode, bt, shp, L, s);
     else return hipErrorInvalidValue;
   }

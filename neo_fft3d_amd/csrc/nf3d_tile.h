@@ -524,7 +524,7 @@ __global__ void __launch_bounds__(NT) k_tile(TileLaunch L_arg) {
       // wavefront, highest blocks first: spectral block b overlaps only the
       // gather stage of blocks b and b+1, so a round writes over stages its own
       // round or earlier rounds have read (one barrier per round)
-      static_assert(BW == 32 && BH == 32, "matrix-core DFT: 32x32 blocks");
+      static_assert(BW == BH && (BW == 32 || BW == 64), "matrix-core DFT: 32x32 or 64x64 blocks");
       static_assert(LSPEC >= LA && 2 * LA - LSPEC >= (BPW - 1) * (LSPEC - LA), "spectra overlap stages b, b+1 only");
       constexpr int NW = NT / 64, MB = (BPW + NW - 1) / NW;
       const int wave = __builtin_amdgcn_readfirstlane(t >> 6), lane = t & 63;
@@ -533,9 +533,9 @@ __global__ void __launch_bounds__(NT) k_tile(TileLaunch L_arg) {
 #pragma unroll 1
       for (int q = 0; q < MB; q++) {
         const int b = nblk - 1 - (q * NW + wave);  // uniform in the wavefront
-        DftIn din;
+        DftIn<BW> din;
         if (b >= 0)
-          din = dft32_fwd_load(
+          din = dft_fwd_load<BW>(
               [&](int r, int c) {
                 const float2 v = stage[b * LA + (r >> 1) * SA + c];
                 return (r & 1) ? v.y : v.x;
@@ -543,7 +543,7 @@ __global__ void __launch_bounds__(NT) k_tile(TileLaunch L_arg) {
               sx, lane);
         __syncthreads();  // the round's stages read
         if (b >= 0)
-          dft32_fwd_compute(din, sx, T, lane, [&](int r, int k, float2 zz) { spec[b * LSPEC + r * SR + k] = zz; });
+          dft_fwd_compute<BW>(din, sx, T, lane, [&](int r, int k, float2 zz) { spec[b * LSPEC + r * SR + k] = zz; });
       }
     } else {
       // ---- 2. row r2c, two real rows per lane
@@ -699,11 +699,11 @@ __global__ void __launch_bounds__(NT) k_tile(TileLaunch L_arg) {
 #pragma unroll 1
       for (int q = 0; q < MB; q++) {
         const int b = q * NW + wave;  // uniform in the wavefront
-        DftSpec dsp;
-        if (b < nblk) dsp = dft32_inv_load([&](int r, int k) { return spec[b * LSPEC + r * SR + k]; }, sz, lane);
+        DftSpec<BW> dsp;
+        if (b < nblk) dsp = dft_inv_load<BW>([&](int r, int k) { return spec[b * LSPEC + r * SR + k]; }, sz, lane);
         __syncthreads();  // the round's spectra read
         if (b < nblk)
-          dft32_inv_compute(dsp, sz, T, lane, [&](int r, int c0, f4 x) {
+          dft_inv_compute<BW>(dsp, sz, T, lane, [&](int r, int c0, f4 x) {
             float* d = syn + b * LSYN + r * SYNW + c0;
             d[0] = x[0];
             d[1] = x[1];
@@ -825,11 +825,11 @@ hipError_t launch_tile_modes(int mode, int bt, int onchip, const TileLaunch& L, 
   return hipGetLastError();
 }
 
-// mf: the matrix-core DFT variant (32x32 only)
+// mf: the matrix-core DFT variant (32x32 and 64x64 blocks)
 template <int BW, int BH>
 hipError_t launch_tile_shape(int mode, int bt, int onchip, int mf, const TileLaunch& L, hipStream_t s) {
   if (!mf) return launch_tile_modes<BW, BH, false>(mode, bt, onchip, L, s);
-  if constexpr (BW == 32 && BH == 32) {
+  if constexpr (BW == BH && (BW == 32 || BW == 64)) {
     for (int k = 0; k < L.njobs; k++)
       if (!L.job[k].dft_tab) return hipErrorInvalidValue;
     return launch_tile_modes<BW, BH, true>(mode, bt, onchip, L, s);

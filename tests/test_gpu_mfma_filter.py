@@ -1,9 +1,11 @@
 """The filter with its 2D transforms as DFT-matrix products on the matrix
-cores (FP16 hi/lo split operands, FP32 accumulation; `dft=1`, 32x32 blocks)
+cores (FP16 hi/lo split operands, FP32 accumulation; `dft=1`, 32x32 blocks in
+the frame loop and the per-frame kernel, 64x64 blocks in the per-frame kernel)
 against the FP64 oracle, with the same bars as the FFT form: integer formats
 within +-1 LSB with >= 99.9% of pixels exact, float within 1e-4.  The 10-bit
 4:2:0 cases follow the precision-form workload (bt=3 Wiener and bt=0 Kalman at
-32/16) and report each plane's error against FP64 for both transform forms."""
+32/16 and 64/16) and report each plane's error against FP64 for both
+transform forms."""
 import numpy as np
 import pytest
 
@@ -21,6 +23,10 @@ MODES = [
     "sigma=3;bt=-1;bw=32;bh=32;sharpen=1",
     "sigma=3;bt=2;bw=32;bh=32;ow=16;oh=16;sharpen=0.3;dehalo=0.5",
     "sigma=3;bt=1;bw=32;bh=32;pfactor=1;pshow=1",
+    "sigma=3;bt=3;bw=64;bh=64;ow=16;oh=16",
+    "sigma=3;bt=0;bw=64;bh=64;ow=16;oh=16;kratio=2",
+    "sigma=3;bt=1;bw=64;bh=64;ow=16;oh=16;sharpen=0.5",
+    "sigma=3;bt=5;bw=64;bh=64;ow=16;oh=16;dehalo=1",
 ]
 
 
@@ -43,7 +49,9 @@ def test_yuv_p16_mfma(product, oracle):
 
 
 @pytest.mark.parametrize("args", ["sigma=3;bt=3;bw=32;bh=32;ow=16;oh=16",
-                                  "sigma=3;bt=0;kratio=2;bw=32;bh=32;ow=16;oh=16"])
+                                  "sigma=3;bt=0;kratio=2;bw=32;bh=32;ow=16;oh=16",
+                                  "sigma=3;bt=3;bw=64;bh=64;ow=16;oh=16",
+                                  "sigma=3;bt=0;kratio=2;bw=64;bh=64;ow=16;oh=16"])
 def test_yuv420p10_precision_form_vs_fp64(product, oracle, args):
     """Both transform forms against FP64, plane by plane."""
     clip = make_clip("yuv", 10, False, width=320, height=192, nframes=6, seed=21, noise=12.0)
@@ -65,5 +73,5 @@ def test_yuv420p10_precision_form_vs_fp64(product, oracle, args):
 def test_rejects_other_block_shapes(product):
     from harness import FilterError
     clip = make_clip("gray", 8, False, width=160, height=128, nframes=3, seed=2)
-    with pytest.raises(FilterError, match="32x32"):
+    with pytest.raises(FilterError, match="32x32 and 64x64"):
         product.open("sigma=3;bw=48;bh=48;dft=1", clip)
