@@ -1557,7 +1557,9 @@ int nf3d_run_host_batch(const nf3d_run_item* items, int n, char* err, size_t err
         P->e = e;
         P->row = (size_t)e->framewidth * e->pix;
         P->rows = e->frameheight;
-        P->pitch = (P->row + 255) / 256 * 256;
+        // rows that are a multiple of 64 bytes keep their width as the slot
+        // pitch, so a frame (and a run of frames) is one contiguous copy
+        P->pitch = P->row % 64 == 0 ? P->row : (P->row + 255) / 256 * 256;
         const size_t slot = P->pitch * P->rows;
         P->d_in = dev_alloc<uint8_t>(slot * kInSlots);
         P->d_out = dev_alloc<uint8_t>(slot * 2 * kChunk);
@@ -1615,15 +1617,28 @@ int nf3d_run_host_batch(const nf3d_run_item* items, int n, char* err, size_t err
       if (c >= 2) check(hipStreamWaitEvent(s_up, ev_run[c - 2], 0), "hipStreamWaitEvent");
       for (int i = 0; i < n; i++) {
         HostPlane& P = *pl[i];
-        for (int k = lo; k <= hi; k++) {
-          if (pin_src[i])
+        const size_t frame = P.pitch * P.rows;
+        const bool dense = pin_src[i] && (size_t)items[i].src_stride == P.pitch;
+        for (int k = lo; k <= hi;) {
+          // a run of frames adjacent both in the source and in the device
+          // slots goes up as one copy
+          int k1 = k + 1;
+          if (!pin_src[i] || dense) {
+            while (k1 <= hi && (k1 % kInSlots) != 0 &&
+                   (!pin_src[i] || static_cast<const uint8_t*>(items[i].src[k1]) ==
+                                       static_cast<const uint8_t*>(items[i].src[k1 - 1]) + frame))
+              k1++;
+          }
+          if (pin_src[i] && !dense)
             check(hipMemcpy2DAsync(const_cast<void*>(P.dsrc[k]), P.pitch, items[i].src[k], (size_t)items[i].src_stride,
                                    P.row, P.rows, hipMemcpyHostToDevice, s_up),
                   "upload");
           else
-            check(hipMemcpyAsync(const_cast<void*>(P.dsrc[k]), P.h_in[buf] + (size_t)(k - lo) * P.pitch * P.rows,
-                                 P.pitch * P.rows, hipMemcpyHostToDevice, s_up),
+            check(hipMemcpyAsync(const_cast<void*>(P.dsrc[k]),
+                                 pin_src[i] ? items[i].src[k] : P.h_in[buf] + (size_t)(k - lo) * frame,
+                                 frame * (size_t)(k1 - k), hipMemcpyHostToDevice, s_up),
                   "upload");
+          k = k1;
         }
       }
       check(hipEventRecord(ev_up[c], s_up), "hipEventRecord");
@@ -1664,20 +1679,35 @@ int nf3d_run_host_batch(const nf3d_run_item* items, int n, char* err, size_t err
       for (int i = 0; i < n; i++) {
         HostPlane& P = *pl[i];
         nf3d_engine* e = P.e;
-        for (int k = a; k < a + cnt; k++) {
-          if (pin_dst[i])
-            check(hipMemcpy2DAsync(static_cast<uint8_t*>(items[i].dst[k]) + (size_t)e->tc * items[i].dst_stride +
-                                       (size_t)e->lc * e->pix,
-                                   (size_t)items[i].dst_stride,
-                                   static_cast<const uint8_t*>(P.ddst[k]) + (size_t)e->tc * P.pitch +
-                                       (size_t)e->lc * e->pix,
-                                   P.pitch, (size_t)e->Wc * e->pix, e->Hc, hipMemcpyDeviceToHost, s_down),
+        const size_t frame = P.pitch * P.rows;
+        // page-locked destination without crop and with the slots' pitch:
+        // whole frames, a run of adjacent ones per copy
+        const bool dense = pin_dst[i] && (size_t)items[i].dst_stride == P.pitch && e->Wc == e->framewidth &&
+                           e->Hc == e->frameheight;
+        if (!pin_dst[i] || dense) {
+          // the chunk's output slots are adjacent (slot k % (2 kChunk), chunks start at multiples of kChunk
+          // from `first`, so a run stops where the slot index wraps)
+          for (int k = a; k < a + cnt;) {
+            int k1 = k + 1;
+            while (k1 < a + cnt && (k1 % (2 * kChunk)) != 0 &&
+                   (!pin_dst[i] ||
+                    static_cast<uint8_t*>(items[i].dst[k1]) == static_cast<uint8_t*>(items[i].dst[k1 - 1]) + frame))
+              k1++;
+            check(hipMemcpyAsync(pin_dst[i] ? items[i].dst[k] : P.h_out[c & 1] + (size_t)(k - a) * frame, P.ddst[k],
+                                 frame * (size_t)(k1 - k), hipMemcpyDeviceToHost, s_down),
                   "download");
-          else
-            check(hipMemcpyAsync(P.h_out[c & 1] + (size_t)(k - a) * P.pitch * P.rows, P.ddst[k], P.pitch * P.rows,
-                                 hipMemcpyDeviceToHost, s_down),
-                  "download");
+            k = k1;
+          }
+          continue;
         }
+        for (int k = a; k < a + cnt; k++)
+          check(hipMemcpy2DAsync(static_cast<uint8_t*>(items[i].dst[k]) + (size_t)e->tc * items[i].dst_stride +
+                                     (size_t)e->lc * e->pix,
+                                 (size_t)items[i].dst_stride,
+                                 static_cast<const uint8_t*>(P.ddst[k]) + (size_t)e->tc * P.pitch +
+                                     (size_t)e->lc * e->pix,
+                                 P.pitch, (size_t)e->Wc * e->pix, e->Hc, hipMemcpyDeviceToHost, s_down),
+                "download");
       }
       check(hipEventRecord(ev_down[c], s_down), "hipEventRecord");
       // meanwhile: the next chunk's frames up, the previous chunk's out
