@@ -16,7 +16,11 @@
 //                         during the MFMAs;
 //                         epilogues: bias + GELU, bias, bias + exp (PSD),
 //                         GELU backward, plain FP32 (weight gradients)
-//   k_features            log-power features of compact block spectra
+//   k_fwd<EPI, KS>        inference layers: fragments loaded straight from
+//                         memory against the transposed weights, K split
+//                         over the workgroup's waves, coalesced epilogue
+//   k_transpose           the transposed BF16 weights, after each update
+//   k_features           log-power features of compact block spectra
 //   k_whittle             Whittle loss y + t e^-y and its gradient
 //   k_colsum              bias gradients (column sums, two passes)
 //   k_adam                Adam update of the FP32 master and its BF16 copy
@@ -343,6 +347,339 @@ hipError_t gemm_splitk(GemmArgs g, float* ws, hipStream_t s) {
   return hipGetLastError();
 }
 
+// ------------------------------------------------------------ inference
+// k_fwd<EPI, KS>: one layer of inference, C(m, n) = sum_k A(m, k) Bt(n, k),
+// both operands k-contiguous (Bt is the layer's weight matrix transposed),
+// so every MFMA fragment is one 16-byte load straight from memory: no LDS
+// staging and no barrier inside the K loop.  A workgroup covers 32 rows x
+// 64 * (4 / KS) columns; its four waves hold 32 x 64 tiles (2 x 4 MFMA
+// tiles) and split K in KS interleaved phases of 32-wide steps, two steps in
+// flight per wave.  The partial tiles meet in LDS, are added in phase order
+// (deterministic, and independent of M), and leave through the epilogue in
+// row order (coalesced stores).
+struct FwdArgs {
+  const __bf16* A;   // [M][lda]
+  const __bf16* Bt;  // [N][ldb]
+  long long lda, ldb;
+  int M, N, K;
+  const float* bias;
+  float* C;  // EPI_BIAS / EPI_BIAS_EXP
+  long long ldc;
+  __bf16* H;  // EPI_BIAS_GELU
+  long long ldh;
+  float scale;
+};
+
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+template <int EPI, int KS>
+__global__ void __launch_bounds__(NT) k_fwd(FwdArgs g) {
+  constexpr int CW = 4 / KS, TNW = 64 * CW, LDR = 68;
+  __shared__ float red[4][32 * LDR];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int cw = w % CW, kw = w / CW;
+  const int m0 = blockIdx.y * 32, nb = blockIdx.x * TNW, n0 = nb + cw * 64;
+  const int r16 = lane & 15, kg = (lane >> 4) * 8;
+  const __bf16* ap[2];
+  const __bf16* bp[4];
+#pragma unroll
+  for (int i = 0; i < 2; i++) ap[i] = g.A + (long long)min(m0 + i * 16 + r16, g.M - 1) * g.lda + kg;
+#pragma unroll
+  for (int j = 0; j < 4; j++) bp[j] = g.Bt + (long long)min(n0 + j * 16 + r16, g.N - 1) * g.ldb + kg;
+  // the bias joins the first K phase's partial tile before the exchange, so
+  // the store loop below holds no load (a load there would wait for the
+  // stores in flight)
+  float bj[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) bj[j] = kw == 0 ? g.bias[min(n0 + j * 16 + r16, g.N - 1)] : 0.0f;
+  const int ns = (g.K + 31) / 32;
+  f32x4 acc[2][4];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  u32x4 a0[2], b0[4], a1[2], b1[4];
+  // Branch-free loads, their values used only by the MFMAs (a load under a
+  // branch, or a select right after it, makes the compiler wait for the
+  // loads in flight): rows past M read row M-1 and columns past N row N-1
+  // (neither is stored); a k-chunk past K reads the row's last chunk and
+  // its A fragment is zeroed at the MFMA (a zero times finite weights adds
+  // nothing).
+  const int klim = g.K - 8 - kg;  // last valid chunk offset of this lane (K >= 8)
+  auto load = [&](u32x4* a, u32x4* b, int s) {
+    const int ko = max(min(s * 32, klim), -kg);
+#pragma unroll
+    for (int i = 0; i < 2; i++) a[i] = *reinterpret_cast<const u32x4*>(ap[i] + ko);
+#pragma unroll
+    for (int j = 0; j < 4; j++) b[j] = *reinterpret_cast<const u32x4*>(bp[j] + ko);
+  };
+  auto mma = [&](const u32x4* a, const u32x4* b, int s) {
+    const bool ok = s * 32 <= klim;
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      const bf16x8 av = __builtin_bit_cast(bf16x8, ok ? a[i] : u32x4{0, 0, 0, 0});
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, __builtin_bit_cast(bf16x8, b[j]), acc[i][j], 0, 0, 0);
+    }
+  };
+  load(a0, b0, kw);
+  __builtin_amdgcn_sched_barrier(0);
+  load(a1, b1, kw + KS);
+  __builtin_amdgcn_sched_barrier(0);
+  // program order kept (sched_barrier): each buffer's loads go out right
+  // after its MFMAs, so the MFMAs of one buffer wait only for that buffer
+  for (int s = kw; s < ns; s += 2 * KS) {
+    mma(a0, b0, s);
+    __builtin_amdgcn_sched_barrier(0);
+    load(a0, b0, s + 2 * KS);
+    __builtin_amdgcn_sched_barrier(0);
+    mma(a1, b1, s + KS);
+    __builtin_amdgcn_sched_barrier(0);
+    load(a1, b1, s + 3 * KS);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  // lane holds rows 4 * (lane / 16) + r of each 16x16 tile, column lane % 16
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) red[w][(i * 16 + (lane >> 4) * 4 + r) * LDR + j * 16 + r16] = acc[i][j][r] + bj[j];
+  __syncthreads();
+  // each thread keeps one column (NT is a multiple of TNW)
+  constexpr int RS = NT / TNW;
+  const int col = t % TNW, n = nb + col, cc = col % 64;
+#pragma unroll
+  for (int q = 0; q < 32 / RS; q++) {
+    const int row = t / TNW + q * RS, m = m0 + row;
+    if (m >= g.M || n >= g.N) continue;
+    float v = red[col / 64][row * LDR + cc];
+#pragma unroll
+    for (int p = 1; p < KS; p++) v += red[col / 64 + p * CW][row * LDR + cc];
+    const float z = v;
+    if constexpr (EPI == EPI_BIAS_GELU) {
+      g.H[(long long)m * g.ldh + n] = (__bf16)gelu(z);
+    } else if constexpr (EPI == EPI_BIAS) {
+      g.C[(long long)m * g.ldc + n] = z;
+    } else {
+      static_assert(EPI == EPI_BIAS_EXP, "inference epilogues: bias + GELU, bias, bias + exp");
+      g.C[(long long)m * g.ldc + n] = expf(fminf(z, 80.0f)) * g.scale;
+    }
+  }
+}
+
+// k_fwdp<EPI>: layers with K <= 256 (the hidden layer's width).  Each wave
+// keeps its 64-column slice of the transposed weights in registers (8 steps
+// of 32 k x 4 column tiles of 16, 128 VGPRs) for the whole kernel and takes
+// 32-row tiles of A in turn (blockIdx.y, blockIdx.y + gridDim.y, ...): the
+// weights are read once per wave instead of once per row tile.  A
+// workgroup's four waves hold four adjacent slices (256 columns) and read
+// the same A rows.  A tile's result leaves through the wave's own LDS
+// region: one column per lane, one row per store (coalesced), the bias
+// added on the way.  No workgroup barrier: waves whose slice lies past N
+// leave at once.
+template <int EPI>
+__global__ void __launch_bounds__(NT) k_fwdp(FwdArgs g) {
+  constexpr int LDR = 68;
+  __shared__ float tile[4][32 * LDR];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int n0 = (blockIdx.x * 4 + w) * 64;
+  if (n0 >= g.N) return;
+  const int r16 = lane & 15, kg = (lane >> 4) * 8;
+  const int klim = g.K - 8 - kg;  // last valid chunk offset of this lane (K >= 8)
+  u32x4 bf[8][4];
+#pragma unroll
+  for (int s = 0; s < 8; s++) {
+    const int ko = max(min(s * 32, klim), -kg);
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      bf[s][j] = *reinterpret_cast<const u32x4*>(g.Bt + (long long)min(n0 + j * 16 + r16, g.N - 1) * g.ldb + kg + ko);
+  }
+  const int nc = n0 + lane;  // this lane's output column in the epilogue
+  const float bias = g.bias[min(nc, g.N - 1)];
+  float* T = tile[w];
+  const int nrt = (g.M + 31) / 32;
+  for (int rt = blockIdx.y; rt < nrt; rt += gridDim.y) {
+    const int m0 = rt * 32;
+    u32x4 af[8][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      const __bf16* ap = g.A + (long long)min(m0 + i * 16 + r16, g.M - 1) * g.lda + kg;
+#pragma unroll
+      for (int s = 0; s < 8; s++) af[s][i] = *reinterpret_cast<const u32x4*>(ap + max(min(s * 32, klim), -kg));
+    }
+    f32x4 acc[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+      const bool ok = s * 32 <= klim;  // k past K: the A fragment is zeroed
+#pragma unroll
+      for (int i = 0; i < 2; i++) {
+        const bf16x8 av = __builtin_bit_cast(bf16x8, ok ? af[s][i] : u32x4{0, 0, 0, 0});
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, __builtin_bit_cast(bf16x8, bf[s][j]), acc[i][j], 0, 0,
+                                                               0);
+      }
+    }
+    // lane holds rows 4 * (lane / 16) + r of each 16x16 tile, column lane % 16
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) T[(i * 16 + (lane >> 4) * 4 + r) * LDR + j * 16 + r16] = acc[i][j][r];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (nc < g.N) {
+#pragma unroll 4
+      for (int row = 0; row < 32; row++) {
+        const int m = m0 + row;
+        if (m >= g.M) break;
+        const float z = T[row * LDR + lane] + bias;
+        if constexpr (EPI == EPI_BIAS_GELU) {
+          g.H[(long long)m * g.ldh + nc] = (__bf16)gelu(z);
+        } else if constexpr (EPI == EPI_BIAS) {
+          g.C[(long long)m * g.ldc + nc] = z;
+        } else {
+          static_assert(EPI == EPI_BIAS_EXP, "inference epilogues: bias + GELU, bias, bias + exp");
+          g.C[(long long)m * g.ldc + nc] = expf(fminf(z, 80.0f)) * g.scale;
+        }
+      }
+    }
+    // the next tile's LDS writes come after every lane's reads
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// k_fwdw<EPI>: layers with 256 < K <= 1280 (the input layer: bh * (bw/2+1)
+// log-power features).  A workgroup of eight waves covers 32 rows x 64
+// columns; wave p takes the 32-wide k steps p, p + 8, ..., at most five,
+// and issues all of their loads at once (one memory round trip per wave
+// instead of one per step).  The eight partial tiles meet in LDS and are
+// added in phase order (deterministic, independent of M); the bias joins
+// phase 0's tile; one column per thread in the coalesced epilogue.
+constexpr int NTW = 512, WSTEPS = 5;
+template <int EPI>
+__global__ void __launch_bounds__(NTW) k_fwdw(FwdArgs g) {
+  constexpr int LDR = 68;
+  __shared__ float red[8][32 * LDR];
+  const int t = threadIdx.x, lane = t & 63, p = t >> 6;
+  const int m0 = blockIdx.y * 32, n0 = blockIdx.x * 64;
+  const int r16 = lane & 15, kg = (lane >> 4) * 8;
+  const int ns = (g.K + 31) / 32, klim = g.K - 8 - kg;
+  u32x4 af[WSTEPS][2], bf[WSTEPS][4];
+#pragma unroll
+  for (int q = 0; q < WSTEPS; q++) {
+    const int ko = max(min((p + 8 * q) * 32, klim), -kg);
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+      af[q][i] = *reinterpret_cast<const u32x4*>(g.A + (long long)min(m0 + i * 16 + r16, g.M - 1) * g.lda + kg + ko);
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      bf[q][j] = *reinterpret_cast<const u32x4*>(g.Bt + (long long)min(n0 + j * 16 + r16, g.N - 1) * g.ldb + kg + ko);
+  }
+  float bj[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) bj[j] = p == 0 ? g.bias[min(n0 + j * 16 + r16, g.N - 1)] : 0.0f;
+  f32x4 acc[2][4];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int q = 0; q < WSTEPS; q++) {
+    const int s = p + 8 * q;
+    const bool ok = s < ns && s * 32 <= klim;  // steps past K add zeros
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      const bf16x8 av = __builtin_bit_cast(bf16x8, ok ? af[q][i] : u32x4{0, 0, 0, 0});
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, __builtin_bit_cast(bf16x8, bf[q][j]), acc[i][j], 0, 0,
+                                                             0);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) red[p][(i * 16 + (lane >> 4) * 4 + r) * LDR + j * 16 + r16] = acc[i][j][r] + bj[j];
+  __syncthreads();
+  const int col = t & 63, n = n0 + col;
+#pragma unroll
+  for (int q = 0; q < 32 * 64 / NTW; q++) {
+    const int row = (t >> 6) + q * (NTW / 64), m = m0 + row;
+    if (m >= g.M || n >= g.N) continue;
+    float v = red[0][row * LDR + col];
+#pragma unroll
+    for (int ph = 1; ph < 8; ph++) v += red[ph][row * LDR + col];
+    if constexpr (EPI == EPI_BIAS_GELU) {
+      g.H[(long long)m * g.ldh + n] = (__bf16)gelu(v);
+    } else if constexpr (EPI == EPI_BIAS) {
+      g.C[(long long)m * g.ldc + n] = v;
+    } else {
+      static_assert(EPI == EPI_BIAS_EXP, "inference epilogues: bias + GELU, bias, bias + exp");
+      g.C[(long long)m * g.ldc + n] = expf(fminf(v, 80.0f)) * g.scale;
+    }
+  }
+}
+
+// K <= 256: weights held in registers (k_fwdp), about 512 workgroups;
+// K <= 1280: eight K phases with every load in flight at once (k_fwdw);
+// otherwise K phases per layer, enough that each wave still has four or
+// more steps (k_fwd).
+template <int EPI>
+hipError_t fwd(const FwdArgs& g, hipStream_t s) {
+  if (g.K <= 256) {
+    const int gx = (g.N + 255) / 256, nrt = (g.M + 31) / 32;
+    const int gy = std::max(1, std::min(nrt, (512 + gx - 1) / gx));
+    hipLaunchKernelGGL((k_fwdp<EPI>), dim3(gx, gy), dim3(NT), 0, s, g);
+    return hipGetLastError();
+  }
+  if (g.K <= 8 * WSTEPS * 32) {
+    hipLaunchKernelGGL((k_fwdw<EPI>), dim3((g.N + 63) / 64, (g.M + 31) / 32), dim3(NTW), 0, s, g);
+    return hipGetLastError();
+  }
+  const int ns = (g.K + 31) / 32;
+  const int KS = ns >= 16 ? 4 : ns >= 8 ? 2 : 1;
+  const dim3 rows(1, (g.M + 31) / 32);
+  if (KS == 4) {
+    hipLaunchKernelGGL((k_fwd<EPI, 4>), dim3((g.N + 63) / 64, rows.y), dim3(NT), 0, s, g);
+  } else if (KS == 2) {
+    hipLaunchKernelGGL((k_fwd<EPI, 2>), dim3((g.N + 127) / 128, rows.y), dim3(NT), 0, s, g);
+  } else {
+    hipLaunchKernelGGL((k_fwd<EPI, 1>), dim3((g.N + 255) / 256, rows.y), dim3(NT), 0, s, g);
+  }
+  return hipGetLastError();
+}
+
+// dst[n][k] = src[k][n] for the three weight matrices at once (blockIdx.z),
+// through 32 x 32 LDS tiles.
+struct TransposeSet {
+  const __bf16* src[3];
+  __bf16* dst[3];
+  int K[3], N[3];
+};
+__global__ void __launch_bounds__(256) k_transpose(TransposeSet ts) {
+  __shared__ __bf16 tile[32][34];
+  const int z = blockIdx.z, K = ts.K[z], N = ts.N[z];
+  const int k0 = blockIdx.y * 32, n0 = blockIdx.x * 32;
+  if (k0 >= K || n0 >= N) return;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int r = ty; r < 32; r += 8)
+    if (k0 + r < K && n0 + tx < N) tile[r][tx] = ts.src[z][(long long)(k0 + r) * N + n0 + tx];
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8)
+    if (n0 + r < N && k0 + tx < K) ts.dst[z][(long long)(n0 + r) * K + k0 + tx] = tile[tx][r];
+}
+
 GemmArgs args(const __bf16* A, long long lda, const __bf16* B, long long ldb, int M, int N, int K) {
   GemmArgs g{};
   g.A = A;
@@ -394,6 +731,10 @@ struct nf3d_mlp {
   size_t o_w1 = 0, o_b1 = 0, o_w2 = 0, o_b2 = 0, o_w3 = 0, o_b3 = 0, np = 0;
   float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
   __bf16* pb = nullptr;
+  // the three weight matrices transposed ([N][K], k contiguous), BF16: the
+  // inference operands, rewritten whenever pb is
+  __bf16* pbt = nullptr;
+  size_t t_w1 = 0, t_w2 = 0, t_w3 = 0;
   float *mu = nullptr, *isd = nullptr;
   float eps = 1e-10f;
   // training workspace, max_batch rows
@@ -406,7 +747,7 @@ struct nf3d_mlp {
   ~nf3d_mlp() {
     for (void* q : {(void*)p, (void*)g, (void*)m, (void*)v, (void*)pb, (void*)mu, (void*)isd, (void*)z1, (void*)z2,
                     (void*)y, (void*)gf, (void*)part, (void*)lpart, (void*)h1, (void*)h2, (void*)d3, (void*)d2,
-                    (void*)d1, (void*)wsk})
+                    (void*)d1, (void*)wsk, (void*)pbt})
       (void)hipFree(q);
   }
 
@@ -435,11 +776,16 @@ struct nf3d_mlp {
     m = dalloc<float>(np);
     v = dalloc<float>(np);
     pb = dalloc<__bf16>(np);
+    t_w1 = 0;
+    t_w2 = (size_t)ni * h;
+    t_w3 = t_w2 + (size_t)h * h;
+    pbt = dalloc<__bf16>(t_w3 + (size_t)h * no);
     mcheck(hipMemset(p, 0, np * 4), "hipMemset");
     mcheck(hipMemset(g, 0, np * 4), "hipMemset");
     mcheck(hipMemset(m, 0, np * 4), "hipMemset");
     mcheck(hipMemset(v, 0, np * 4), "hipMemset");
     mcheck(hipMemset(pb, 0, np * 2), "hipMemset");
+    mcheck(hipMemset(pbt, 0, (t_w3 + (size_t)h * no) * 2), "hipMemset");
     mu = dalloc<float>(ni);
     isd = dalloc<float>(ni);
     std::vector<float> zero(ni, 0.0f), one(ni, 1.0f);
@@ -460,9 +806,28 @@ struct nf3d_mlp {
     d1 = dalloc<__bf16>((size_t)mb * h);
   }
 
+  // pbt from pb, on stream s
+  void transpose_weights(hipStream_t s) {
+    TransposeSet ts{};
+    const int Ks[3] = {n_in, nh, nh}, Ns[3] = {nh, nh, n_out};
+    const size_t from[3] = {o_w1, o_w2, o_w3}, to[3] = {t_w1, t_w2, t_w3};
+    int gk = 0, gn = 0;
+    for (int z = 0; z < 3; z++) {
+      ts.src[z] = pb + from[z];
+      ts.dst[z] = pbt + to[z];
+      ts.K[z] = Ks[z];
+      ts.N[z] = Ns[z];
+      gk = std::max(gk, (Ks[z] + 31) / 32);
+      gn = std::max(gn, (Ns[z] + 31) / 32);
+    }
+    hipLaunchKernelGGL(k_transpose, dim3(gn, gk, 3), dim3(256), 0, s, ts);
+    mcheck(hipGetLastError(), "k_transpose");
+  }
+
   void params_changed() {
     hipLaunchKernelGGL(k_to_bf16, dim3(blocks((long long)np, 256)), dim3(256), 0, nullptr, p, pb, (long long)np);
     mcheck(hipGetLastError(), "k_to_bf16");
+    transpose_weights(nullptr);
     mcheck(hipDeviceSynchronize(), "params");
   }
 
@@ -476,23 +841,13 @@ struct nf3d_mlp {
   // Inference: log-PSD (exp_out = 0) or exp(log-PSD) * scale into out [n][n_out].
   void predict(const __bf16* X, int n, float* out, int exp_out, float scale, __bf16* w1, __bf16* w2,
                hipStream_t s) {
-    GemmArgs a = args(X, n_in, pb + o_w1, nh, n, nh, n_in);
-    a.bias = p + o_b1;
-    a.H = w1;
-    a.ldh = nh;
-    mcheck(gemm<false, false, EPI_BIAS_GELU>(a, s), "layer 1");
-    a = args(w1, nh, pb + o_w2, nh, n, nh, nh);
-    a.bias = p + o_b2;
-    a.H = w2;
-    a.ldh = nh;
-    mcheck(gemm<false, false, EPI_BIAS_GELU>(a, s), "layer 2");
-    a = args(w2, nh, pb + o_w3, n_out, n, n_out, nh);
-    a.bias = p + o_b3;
-    a.C = out;
-    a.ldc = n_out;
-    a.scale = scale;
-    if (exp_out) mcheck(gemm<false, false, EPI_BIAS_EXP>(a, s), "layer 3");
-    else mcheck(gemm<false, false, EPI_BIAS>(a, s), "layer 3");
+    FwdArgs a{X, pbt + t_w1, n_in, n_in, n, nh, n_in, p + o_b1, nullptr, 0, w1, nh, 1.0f};
+    mcheck(fwd<EPI_BIAS_GELU>(a, s), "layer 1");
+    a = FwdArgs{w1, pbt + t_w2, nh, nh, n, nh, nh, p + o_b2, nullptr, 0, w2, nh, 1.0f};
+    mcheck(fwd<EPI_BIAS_GELU>(a, s), "layer 2");
+    a = FwdArgs{w2, pbt + t_w3, nh, nh, n, n_out, nh, p + o_b3, out, n_out, nullptr, 0, scale};
+    if (exp_out) mcheck(fwd<EPI_BIAS_EXP>(a, s), "layer 3");
+    else mcheck(fwd<EPI_BIAS>(a, s), "layer 3");
   }
 
   // Forward with the activations kept, Whittle loss, gradients into g.
@@ -573,6 +928,7 @@ struct nf3d_mlp {
     hipLaunchKernelGGL(k_adam, dim3(blocks((long long)np, 256)), dim3(256), 0, s, p, g, m, v, pb, (long long)np, lr,
                        b1, b2, eps_, c1, c2);
     mcheck(hipGetLastError(), "k_adam");
+    transpose_weights(s);
   }
 };
 
