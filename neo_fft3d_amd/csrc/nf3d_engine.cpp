@@ -36,6 +36,7 @@
 #include "../../include/nf3d.h"
 #include "nf3d_internal.h"
 #include "nf3d_mfma.h"
+#include "nf3d_mlp_batch.h"
 
 using nf3d::PatternJob;
 using nf3d::PlaneJob;
@@ -1301,15 +1302,21 @@ void learned_predictions(const nf3d_run_item* items, const std::vector<nf3d_engi
       e->d_bwork = dev_alloc<uint8_t>((size_t)work);
       e->b_work = work;
     }
+    // one feature launch over the group's spectra (at most RUN_MAX_JOBS)
+    static_assert(nf3d::RUN_MAX_JOBS <= 8, "the feature launch takes up to 8 spectra arrays");
+    const void* specs[nf3d::RUN_MAX_JOBS];
+    int seg_rows[nf3d::RUN_MAX_JOBS];
+    int nseg = 0;
     long long row = 0;
     for (int c2 : group) {
       nf3d_engine* e2 = items[members[c0 + c2]].engine;
-      if (nf3d_mlp_features(e->learned, spectra(c2), e2->nb, static_cast<uint16_t*>(e->d_bx) + row * n, s))
-        fail("learned PSD: feature kernel");
+      specs[nseg] = spectra(c2);
+      seg_rows[nseg++] = e2->nb;
       B.job[c2].J.pattern2d = e->d_bp + row * n;
       B.job[c2].J.pattern3d = e->d_bp + row * n;
       row += e2->nb;
     }
+    if (nf3d_mlp_features_batch(e->learned, specs, seg_rows, nseg, e->d_bx, s)) fail("learned PSD: feature kernel");
     char msg[256] = {0};
     if (nf3d_mlp_predict(e->learned, e->d_bx, (int)rows, e->d_bp, 1, scale, e->d_bwork, s, msg, sizeof msg))
       fail(std::string("learned PSD: ") + msg);

@@ -20,7 +20,7 @@
 //                         memory against the transposed weights, K split
 //                         over the workgroup's waves, coalesced epilogue
 //   k_transpose           the transposed BF16 weights, after each update
-//   k_features           log-power features of compact block spectra
+//   k_features_rows       log-power features of compact block spectra
 //   k_whittle             Whittle loss y + t e^-y and its gradient
 //   k_colsum              bias gradients (column sums, two passes)
 //   k_adam                Adam update of the FP32 master and its BF16 copy
@@ -35,6 +35,7 @@
 #include <vector>
 
 #include "../../include/nf3d.h"
+#include "nf3d_mlp_batch.h"
 
 namespace nf3d_mlp_k {
 
@@ -194,15 +195,30 @@ __global__ void __launch_bounds__(NT) k_gemm(GemmArgs g) {
   }
 }
 
-// x = (ln(|c|^2 + eps) - mu[k]) * isd[k] over n blocks of d compact elements.
-__global__ void k_features(const float2* spec, __bf16* X, long long total, int d, const float* mu, const float* isd,
-                           float eps) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= total) return;
-  const int k = (int)(i % d);
-  const float2 c = spec[i];
-  const float p = c.x * c.x + c.y * c.y + eps;
-  X[i] = (__bf16)((logf(p) - mu[k]) * isd[k]);
+// One workgroup per row (block) of a batch, four coefficients per lane, the
+// row's compact spectrum taken from its segment (up to 8 arrays of
+// consecutive rows, e.g. the planes of a frame):
+// x = (ln(|c|^2 + eps) - mu[k]) * isd[k].
+struct FeatSegs {
+  const float2* spec[8];
+  int row0[9];  // segment s holds rows [row0[s], row0[s + 1])
+  int n;
+};
+__global__ void k_features_rows(FeatSegs sg, __bf16* X, int d, const float* mu, const float* isd, float eps) {
+  const int r = blockIdx.x, k = threadIdx.x * 4;
+  if (k >= d) return;
+  int s = 0;
+  while (s + 1 < sg.n && r >= sg.row0[s + 1]) s++;
+  const float2* src = sg.spec[s] + (long long)(r - sg.row0[s]) * d + k;
+  const float4 a = *reinterpret_cast<const float4*>(src), b = *reinterpret_cast<const float4*>(src + 2);
+  const float4 mv = *reinterpret_cast<const float4*>(mu + k), iv = *reinterpret_cast<const float4*>(isd + k);
+  const float p0 = a.x * a.x + a.y * a.y + eps, p1 = a.z * a.z + a.w * a.w + eps;
+  const float p2 = b.x * b.x + b.y * b.y + eps, p3 = b.z * b.z + b.w * b.w + eps;
+  __bf16 o[4] = {(__bf16)((logf(p0) - mv.x) * iv.x), (__bf16)((logf(p1) - mv.y) * iv.y),
+                 (__bf16)((logf(p2) - mv.z) * iv.z), (__bf16)((logf(p3) - mv.w) * iv.w)};
+  uint2 v;
+  memcpy(&v, o, 8);
+  *reinterpret_cast<uint2*>(X + (long long)r * d + k) = v;
 }
 
 // Rows idx[r] of src into row r of dst (rows of `words` 16-byte words).
@@ -469,78 +485,80 @@ __global__ void __launch_bounds__(NT) k_fwd(FwdArgs g) {
   }
 }
 
-// k_fwdp<EPI>: layers with K <= 256 (the hidden layer's width).  Each wave
-// keeps its 64-column slice of the transposed weights in registers (8 steps
-// of 32 k x 4 column tiles of 16, 128 VGPRs) for the whole kernel and takes
-// 32-row tiles of A in turn (blockIdx.y, blockIdx.y + gridDim.y, ...): the
-// weights are read once per wave instead of once per row tile.  A
-// workgroup's four waves hold four adjacent slices (256 columns) and read
-// the same A rows.  A tile's result leaves through the wave's own LDS
-// region: one column per lane, one row per store (coalesced), the bias
-// added on the way.  No workgroup barrier: waves whose slice lies past N
-// leave at once.
-template <int EPI>
+// k_fwdp<EPI, MI, NJ>: layers with K <= 256 (the hidden layer's width).
+// Each wave keeps its 16*NJ-column slice of the transposed weights in
+// registers (8 steps of 32 k x NJ column tiles of 16) for the whole kernel
+// and takes 16*MI-row tiles of A in turn (blockIdx.y, blockIdx.y +
+// gridDim.y, ...): the weights are read once per wave instead of once per
+// row tile.  A workgroup's four waves hold four adjacent slices and read the
+// same A rows.  A tile's result leaves through the wave's own LDS region in
+// row order (coalesced stores), the bias added on the way.  No workgroup
+// barrier: waves whose slice lies past N leave at once.  The tile shape only
+// changes which wave computes a row, not its arithmetic (each 16x16 output
+// tile is the same eight MFMAs in the same order).
+template <int EPI, int MI, int NJ>
 __global__ void __launch_bounds__(NT) k_fwdp(FwdArgs g) {
-  constexpr int LDR = 68;
-  __shared__ float tile[4][32 * LDR];
+  constexpr int TR = 16 * MI, TC = 16 * NJ, LDR = TC + 4;
+  constexpr int RPS = 64 / TC;  // rows per store step (one column per lane)
+  __shared__ float tile[4][TR * LDR];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int n0 = (blockIdx.x * 4 + w) * 64;
+  const int n0 = (blockIdx.x * 4 + w) * TC;
   if (n0 >= g.N) return;
   const int r16 = lane & 15, kg = (lane >> 4) * 8;
   const int klim = g.K - 8 - kg;  // last valid chunk offset of this lane (K >= 8)
-  u32x4 bf[8][4];
+  u32x4 bf[8][NJ];
 #pragma unroll
   for (int s = 0; s < 8; s++) {
     const int ko = max(min(s * 32, klim), -kg);
 #pragma unroll
-    for (int j = 0; j < 4; j++)
+    for (int j = 0; j < NJ; j++)
       bf[s][j] = *reinterpret_cast<const u32x4*>(g.Bt + (long long)min(n0 + j * 16 + r16, g.N - 1) * g.ldb + kg + ko);
   }
-  const int nc = n0 + lane;  // this lane's output column in the epilogue
+  const int cl = lane % TC, nc = n0 + cl;  // this lane's output column in the epilogue
   const float bias = g.bias[min(nc, g.N - 1)];
   float* T = tile[w];
-  const int nrt = (g.M + 31) / 32;
+  const int nrt = (g.M + TR - 1) / TR;
   for (int rt = blockIdx.y; rt < nrt; rt += gridDim.y) {
-    const int m0 = rt * 32;
-    u32x4 af[8][2];
+    const int m0 = rt * TR;
+    u32x4 af[8][MI];
 #pragma unroll
-    for (int i = 0; i < 2; i++) {
+    for (int i = 0; i < MI; i++) {
       const __bf16* ap = g.A + (long long)min(m0 + i * 16 + r16, g.M - 1) * g.lda + kg;
 #pragma unroll
       for (int s = 0; s < 8; s++) af[s][i] = *reinterpret_cast<const u32x4*>(ap + max(min(s * 32, klim), -kg));
     }
-    f32x4 acc[2][4];
+    f32x4 acc[MI][NJ];
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < MI; i++)
 #pragma unroll
-      for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+      for (int j = 0; j < NJ; j++) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int s = 0; s < 8; s++) {
       const bool ok = s * 32 <= klim;  // k past K: the A fragment is zeroed
 #pragma unroll
-      for (int i = 0; i < 2; i++) {
+      for (int i = 0; i < MI; i++) {
         const bf16x8 av = __builtin_bit_cast(bf16x8, ok ? af[s][i] : u32x4{0, 0, 0, 0});
 #pragma unroll
-        for (int j = 0; j < 4; j++)
+        for (int j = 0; j < NJ; j++)
           acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, __builtin_bit_cast(bf16x8, bf[s][j]), acc[i][j], 0, 0,
                                                                0);
       }
     }
     // lane holds rows 4 * (lane / 16) + r of each 16x16 tile, column lane % 16
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < MI; i++)
 #pragma unroll
-      for (int j = 0; j < 4; j++)
+      for (int j = 0; j < NJ; j++)
 #pragma unroll
         for (int r = 0; r < 4; r++) T[(i * 16 + (lane >> 4) * 4 + r) * LDR + j * 16 + r16] = acc[i][j][r];
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     if (nc < g.N) {
 #pragma unroll 4
-      for (int row = 0; row < 32; row++) {
-        const int m = m0 + row;
+      for (int q = 0; q < TR / RPS; q++) {
+        const int row = lane / TC + q * RPS, m = m0 + row;
         if (m >= g.M) break;
-        const float z = T[row * LDR + lane] + bias;
+        const float z = T[row * LDR + cl] + bias;
         if constexpr (EPI == EPI_BIAS_GELU) {
           g.H[(long long)m * g.ldh + nc] = (__bf16)gelu(z);
         } else if constexpr (EPI == EPI_BIAS) {
@@ -555,6 +573,14 @@ __global__ void __launch_bounds__(NT) k_fwdp(FwdArgs g) {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
   }
+}
+
+template <int EPI, int MI, int NJ>
+hipError_t fwdp(const FwdArgs& g, hipStream_t s) {
+  const int gx = (g.N + 64 * NJ - 1) / (64 * NJ), nrt = (g.M + 16 * MI - 1) / (16 * MI);
+  const int gy = std::max(1, std::min(nrt, (512 + gx - 1) / gx));
+  hipLaunchKernelGGL((k_fwdp<EPI, MI, NJ>), dim3(gx, gy), dim3(NT), 0, s, g);
+  return hipGetLastError();
 }
 
 // k_fwdw<EPI>: layers with 256 < K <= 1280 (the input layer: bh * (bw/2+1)
@@ -631,17 +657,19 @@ __global__ void __launch_bounds__(NTW) k_fwdw(FwdArgs g) {
   }
 }
 
-// K <= 256: weights held in registers (k_fwdp), about 512 workgroups;
+// K <= 256: weights held in registers (k_fwdp), at most about 512
+// workgroups;
 // K <= 1280: eight K phases with every load in flight at once (k_fwdw);
 // otherwise K phases per layer, enough that each wave still has four or
 // more steps (k_fwd).
 template <int EPI>
 hipError_t fwd(const FwdArgs& g, hipStream_t s) {
   if (g.K <= 256) {
-    const int gx = (g.N + 255) / 256, nrt = (g.M + 31) / 32;
-    const int gy = std::max(1, std::min(nrt, (512 + gx - 1) / gx));
-    hipLaunchKernelGGL((k_fwdp<EPI>), dim3(gx, gy), dim3(NT), 0, s, g);
-    return hipGetLastError();
+    // the largest wave tile that still gives every SIMD a wave (1,024 waves)
+    auto waves = [&](int mi, int nj) { return (long long)((g.N + 16 * nj - 1) / (16 * nj)) * ((g.M + 16 * mi - 1) / (16 * mi)); };
+    if (waves(2, 4) >= 1024) return fwdp<EPI, 2, 4>(g, s);
+    if (waves(1, 4) >= 1024) return fwdp<EPI, 1, 4>(g, s);
+    return fwdp<EPI, 1, 2>(g, s);
   }
   if (g.K <= 8 * WSTEPS * 32) {
     hipLaunchKernelGGL((k_fwdw<EPI>), dim3((g.N + 63) / 64, (g.M + 31) / 32), dim3(NTW), 0, s, g);
@@ -952,6 +980,32 @@ hipStream_t st(void* s) { return static_cast<hipStream_t>(s); }
 
 }  // namespace
 
+// Features of nseg spectra arrays (rows[s] blocks each) into consecutive
+// rows of x_out, one launch.
+int nf3d_mlp_features_batch(nf3d_mlp* m, const void* const* specs, const int* rows, int nseg, void* x_out,
+                            void* stream) {
+  if (!m || nseg < 1 || nseg > 8) return -1;
+  const int d = m->n_in;
+  if (d % 4 || d / 4 > 1024) return -1;
+  FeatSegs sg{};
+  long long total = 0;
+  for (int s = 0; s < nseg; s++) {
+    if (rows[s] < 0 || (rows[s] > 0 && !specs[s])) return -1;
+    sg.spec[s] = static_cast<const float2*>(specs[s]);
+    sg.row0[s] = (int)total;
+    total += rows[s];
+  }
+  if (total > 0x7fffffff) return -1;
+  sg.row0[nseg] = (int)total;
+  sg.n = nseg;
+  if (!total) return 0;
+  if (!x_out) return -1;
+  const int threads = (d / 4 + 63) / 64 * 64;
+  hipLaunchKernelGGL(k_features_rows, dim3((unsigned)total), dim3(threads), 0, st(stream), sg,
+                     static_cast<__bf16*>(x_out), d, m->mu, m->isd, m->eps);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 // ---------------------------------------------------------------- C ABI
 extern "C" {
 
@@ -1040,12 +1094,7 @@ int nf3d_mlp_set_input_norm(nf3d_mlp* m, const float* mu, const float* isd, floa
 }
 
 int nf3d_mlp_features(nf3d_mlp* m, const void* spec, int n, void* x_out, void* stream) {
-  if (n < 0 || (n > 0 && (!spec || !x_out))) return -1;
-  const long long tot = (long long)n * m->n_in;
-  if (!tot) return 0;
-  hipLaunchKernelGGL(k_features, dim3(blocks(tot, 256)), dim3(256), 0, st(stream), static_cast<const float2*>(spec),
-                     static_cast<__bf16*>(x_out), tot, m->n_in, m->mu, m->isd, m->eps);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
+  return nf3d_mlp_features_batch(m, &spec, &n, 1, x_out, stream);
 }
 
 int nf3d_to_bf16(const float* x, void* out, long long count, void* stream) {
