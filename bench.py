@@ -809,6 +809,9 @@ def main():
     ap.add_argument("--no-precision-form", action="store_true",
                     help="skip the DFT-as-GEMM (MFMA, FP16 split) transform measurement")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-PSD leg")
+    ap.add_argument("--learned-metric-only", action="store_true",
+                    help="of the learned-PSD leg, run only the headline-workload part (training and "
+                         "plain vs learned frames/s; no quality study)")
     ap.add_argument("--learned-frames", type=int, default=96)
     ap.add_argument("--precision-frames", type=int, default=120)
     ap.add_argument("--no-multi-clip", action="store_true", help="skip the 16-clip batched leg")
@@ -839,7 +842,10 @@ def main():
     if not args.no_precision_form:
         line["precision_form"] = precision_form(args)
     if not args.no_learned:
-        line["learned_psd"] = learned_form(args)
+        if args.learned_metric_only:
+            line["learned_psd"] = {"metric_config": learned_on_metric_config(args)}
+        else:
+            line["learned_psd"] = learned_form(args)
     if not args.no_multi_clip:
         line["multi_clip"] = multi_clip(args)
     if not args.no_cpu:
