@@ -310,9 +310,9 @@ def precision_form(args):
 def precision_filter(args):
     """The precision workload's matrix on YUV420P10 3840x2160 (sigma 3 scaled
     to 12 in 10-bit units, noise 12 / 8): bt = 0 (Kalman, kratio 2) and bt = 3,
-    each at 32/16 and 64/16, all planes: frames/s of the FFT form and of the
-    DFT-as-GEMM form (dft=1, matrix cores; the frame loop at 32x32, the
-    per-frame kernel at 64x64) on a resident clip, and how often the two
+    each at 32/16, 48/16 and 64/16, all planes: frames/s of the FFT form and of
+    the DFT-as-GEMM form (dft=1, matrix cores; the frame loop at 32x32, the
+    per-frame kernel at 48x48 and 64x64) on a resident clip, and how often the two
     forms' outputs agree.  (Each form's per-plane error
     against FP64 is measured by the tests.)"""
     import numpy as np
@@ -353,7 +353,7 @@ def precision_filter(args):
         return fps, out
     res = []
     for bt in (0, 3):
-        for bw in (32, 64):
+        for bw in (32, 48, 64):
             fps, out_fft = run(bt, bw, 0)
             row = {"bt": bt, "bw": bw, "ow": 16, "fps_fft": round(fps, 1)}
             fps_m, out_m = run(bt, bw, 1)

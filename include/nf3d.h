@@ -148,7 +148,7 @@ int nf3d_graph_launch(nf3d_graph* graph, void* stream);
 void nf3d_graph_destroy(nf3d_graph* graph);
 
 /* DFT-as-GEMM precision form (no reference counterpart): unnormalised 2D
- * DFTs of n x n real blocks (n = 32 or 64) as dense DFT-matrix products on the
+ * DFTs of n x n real blocks (n = 32, 48 or 64) as dense DFT-matrix products on the
  * matrix cores, FP16 hi/lo split operands (hi*hi + hi*lo + lo*hi) with FP32
  * accumulation.  Forward: in [nblocks][n][n] float -> out [nblocks][n][n/2+1]
  * complex (FFTW r2c layout); inverse: in [nblocks][n][n/2+1] complex -> out
@@ -248,7 +248,7 @@ int nf3d_engine_set_pattern_regions(nf3d_engine* e, int rx, int ry, char* err, s
 /* Precision form of the engine's 2D transforms (no reference counterpart; the
  * reference uses FFTW): mode 0 = FP32 FFT in registers (default), 1 = dense
  * DFT-matrix products on the matrix cores with FP16 hi/lo split operands and
- * FP32 accumulation (32x32 and 64x64 blocks, integer formats: the FP16
+ * FP32 accumulation (32x32, 48x48 and 64x64 blocks, integer formats: the FP16
  * operand scales come from the format's sample range).  Errors: other block
  * shapes, float formats, other modes. */
 int nf3d_engine_set_dft(nf3d_engine* e, int mode, char* err, size_t errlen);

@@ -182,7 +182,7 @@ class Engine:
 
     def set_dft(self, mode: int):
         """2D transform form: 0 = FFT in registers, 1 = DFT-matrix products on
-        the matrix cores (32x32 and 64x64 blocks)."""
+        the matrix cores (32x32, 48x48 and 64x64 blocks)."""
         L = lib()
         L.nf3d_engine_set_dft.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
         err = ctypes.create_string_buffer(256)

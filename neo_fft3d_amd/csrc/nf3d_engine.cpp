@@ -1034,9 +1034,10 @@ struct nf3d_engine {
                       (long long)(tc + Hc) * dst_pitch < (1LL << 31) && 4LL * Hc * Wc < (1LL << 31) &&
                       dst_pitch % pix == 0;
     const bool pshow = ep.pfactor != 0 && ep.pshow;
-    // the matrix-core form of 64x64 blocks runs in the per-frame kernel only
-    const bool mf64 = dft_mode && ep.bw == 64;
-    P.run = P.a <= P.b && fits && !pshow && !mf64 && !(P.mode == nf3d::MODE_SHARPEN && P.shp == nf3d::SHP_NONE);
+    // the matrix-core form of 48x48 and 64x64 blocks runs in the per-frame
+    // kernel only (in the frame loop its operands would not fit the registers)
+    const bool mf_tile = dft_mode && ep.bw != 32;
+    P.run = P.a <= P.b && fits && !pshow && !mf_tile && !(P.mode == nf3d::MODE_SHARPEN && P.shp == nf3d::SHP_NONE);
     if (!P.run) return P;
     // launches of equal length (within one frame), two halves of seam
     // buffers in turn
@@ -1971,7 +1972,7 @@ int nf3d_engine_set_dft(nf3d_engine* e, int mode, char* err, size_t errlen) {
   try {
     if (mode != 0 && mode != 1) fail("dft mode must be 0 (FFT) or 1 (matrix-core DFT)");
     if (mode == 1 && !nf3d::mfma_dft_supported(e->ep.bw, e->ep.bh))
-      fail("the matrix-core DFT supports 32x32 and 64x64 blocks");
+      fail("the matrix-core DFT supports 32x32, 48x48 and 64x64 blocks");
     // operands are scaled to FP16 from the format's range, which float
     // samples do not bound
     if (mode == 1 && e->fmt.is_float) fail("the matrix-core DFT supports integer formats only");

@@ -132,7 +132,7 @@ int tile_blocks_per_wg(int bw, int bh) {
   return 128 / (cols > pairs ? cols : pairs);
 }
 
-bool mfma_dft_supported(int bw, int bh) { return bw == bh && (bw == 32 || bw == 64); }
+bool mfma_dft_supported(int bw, int bh) { return bw == bh && (bw == 32 || bw == 48 || bw == 64); }
 
 hipError_t launch_tile(int mode, int bt, int onchip, int mf, int bw, int bh, const TileLaunch& l, hipStream_t s) {
   if (mf && !mfma_dft_supported(bw, bh)) return hipErrorInvalidValue;

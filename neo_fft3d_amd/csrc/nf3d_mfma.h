@@ -1,4 +1,4 @@
-// N x N block DFTs (N = 32 or 64) as dense DFT-matrix products on the gfx950
+// N x N block DFTs (N = 32, 48 or 64) as dense DFT-matrix products on the gfx950
 // matrix cores, shared by the standalone transforms (nf3d_mfma_dft.hip) and
 // the filter's kernels (k_run / k_tile with the matrix-core DFT).
 //
@@ -21,7 +21,9 @@
 // K2(g, i) = 4g + i for i < 4 and 16 + 4g + (i - 4) for i >= 4: the lane's
 // four values of each D tile, side by side, are then directly its B
 // operand, and the constant A tiles of that product are stored in K2 order.
-// A K of 64 is two products of 32.  With H = N/2 + 1 kept frequencies in
+// A K of 64 is two products of 32; a K of 48 is padded to 64 with zero
+// operands (zero table entries and, for the products of two D-tile chains,
+// a zero fourth row tile).  With H = N/2 + 1 kept frequencies in
 // NC = ceil(H / 16) column tiles, the chains are:
 //   forward  Y = X C        (C: N x 2*16*NC, columns k < H cos, 16*NC + k -sin)
 //            Z = F Y        (F = Fr + i Fi, Fr = cos, Fi = -sin; N x N)
@@ -81,10 +83,16 @@ __device__ __forceinline__ f4 mma3(const H8& a, const H8& b, f4 c) {
 // Tile counts of an N x N block.
 template <int N>
 struct DftShape {
-  static_assert(N == 32 || N == 64, "matrix-core DFT: 32x32 or 64x64 blocks");
+  static_assert(N == 32 || N == 48 || N == 64, "matrix-core DFT: 32x32, 48x48 or 64x64 blocks");
   static constexpr int R = N / 16;              // 16-row tiles of the block
-  static constexpr int KS = N / 32;             // K steps over the block's rows / columns
+  static constexpr int KS = (N + 31) / 32;      // K steps over the block's rows / columns (zero past N)
   static constexpr int H = N / 2 + 1;           // kept frequencies
+  // power of two >= N: intermediate products are scaled by 1/P2 before they
+  // are split, so the scaling is exact (a scale like 1/48 rounds the product
+  // once to FP32 for the low half, while the compiler may round the high
+  // half straight from the exact product: near an FP16 tie the two halves
+  // then disagree by one FP16 step)
+  static constexpr int P2 = N <= 32 ? 32 : 64;
   static constexpr int NC = (H + 15) / 16;      // 16-column tiles of the kept frequencies
   static constexpr int KSI = (16 * NC + 31) / 32;  // K steps over the spectral columns (inverse)
   // constant tiles, [tile][lane] of H8
@@ -124,7 +132,11 @@ __device__ __forceinline__ DftIn<N> dft_fwd_load(X x, float sx, int lane) {
     for (int ks = 0; ks < S::KS; ks++) {
       float v[8];
 #pragma unroll
-      for (int i = 0; i < 8; i++) v[i] = x(16 * m + lr, 32 * ks + 8 * lg + i);
+      for (int i = 0; i < 8; i++) {
+        const int c = 32 * ks + 8 * lg + i;
+        if constexpr (N % 32 == 0) v[i] = x(16 * m + lr, c);
+        else v[i] = c < N ? x(16 * m + lr, c) : 0.0f;  // K padding
+      }
       d.a[m][ks] = split8(v, sx);
     }
   return d;
@@ -139,10 +151,15 @@ __device__ __forceinline__ void dft_fwd_compute(const DftIn<N>& d, float sx, con
   using S = DftShape<N>;
   const int lr = lane & 15, lg = lane >> 4;
   const f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
-  const float back = (float)N / sx;
+  const float back = (float)S::P2 / sx;
 #pragma unroll
   for (int n = 0; n < S::NC; n++) {
-    H2 yr[S::R], yi[S::R];
+    H2 yr[2 * S::KS], yi[2 * S::KS];  // row tiles of Y; past R (K padding) zero
+#pragma unroll
+    for (int m = S::R; m < 2 * S::KS; m++) {
+      yr[m] = split4(zero, 1.0f);
+      yi[m] = yr[m];
+    }
 #pragma unroll
     for (int m = 0; m < S::R; m++) {
       f4 a = zero, b = zero;
@@ -151,8 +168,8 @@ __device__ __forceinline__ void dft_fwd_compute(const DftIn<N>& d, float sx, con
         a = mma3(d.a[m][ks], tile(T, S::C + ks * 2 * S::NC + n, lane), a);
         b = mma3(d.a[m][ks], tile(T, S::C + ks * 2 * S::NC + n + S::NC, lane), b);
       }
-      yr[m] = split4(a, 1.0f / N);  // |Y| <= N max|x|
-      yi[m] = split4(b, 1.0f / N);
+      yr[m] = split4(a, 1.0f / S::P2);  // |Y| <= N max|x|
+      yi[m] = split4(b, 1.0f / S::P2);
     }
 #pragma unroll
     for (int m = 0; m < S::R; m++) {
@@ -197,7 +214,8 @@ __device__ __forceinline__ DftSpec<N> dft_inv_load(Z z, float sz, int lane) {
       float vr[8], vi[8];
 #pragma unroll
       for (int i = 0; i < 8; i++) {
-        const float2 v = k < S::H ? z(32 * ks + 8 * lg + i, k) : make_float2(0.0f, 0.0f);
+        const int r = 32 * ks + 8 * lg + i;
+        const float2 v = k < S::H && (N % 32 == 0 || r < N) ? z(r, k) : make_float2(0.0f, 0.0f);
         vr[i] = v.x;
         vi[i] = v.y;
       }
@@ -216,7 +234,7 @@ __device__ __forceinline__ void dft_inv_compute(const DftSpec<N>& d, float sz, c
   using S = DftShape<N>;
   const int lr = lane & 15, lg = lane >> 4;
   const f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
-  const float back = (float)(2 * N) / sz;
+  const float back = (float)(2 * S::P2) / sz;
 #pragma unroll
   for (int n = 0; n < S::R; n++) {
     H2 br[2 * S::KSI], bi[2 * S::KSI];  // D tiles (spectral-column tile m, this row tile) of Y^T
@@ -237,8 +255,8 @@ __device__ __forceinline__ void dft_inv_compute(const DftSpec<N>& d, float sz, c
         yi = mma3(d.ai[m][ks], fr, yi);
         yi = mma3(d.ar[m][ks], fn, yi);
       }
-      br[m] = split4(yr, 1.0f / (2 * N));  // |Y| <= N max|z|; the c2r weights add 2 x (N/2 + 1)
-      bi[m] = split4(yi, 1.0f / (2 * N));
+      br[m] = split4(yr, 1.0f / (2 * S::P2));  // |Y| <= N max|z|; the c2r weights add 2 x (N/2 + 1)
+      bi[m] = split4(yi, 1.0f / (2 * S::P2));
     }
 #pragma unroll
     for (int m = 0; m < S::R; m++) {

@@ -1,5 +1,5 @@
-// DFT-as-GEMM precision form, standalone: unnormalised 2D DFTs of 32x32 or
-// 64x64 real blocks as dense DFT-matrix products on the gfx950 matrix cores (the block
+// DFT-as-GEMM precision form, standalone: unnormalised 2D DFTs of 32x32,
+// 48x48 or 64x64 real blocks as dense DFT-matrix products on the gfx950 matrix cores (the block
 // functions and the FP16 split are in nf3d_mfma.h), one wavefront per block,
 // four blocks per workgroup; and the host build of the constant tiles, in
 // double, cached per device.
@@ -88,14 +88,16 @@ std::vector<H8> build_tables() {
   const double w = 2.0 * 3.14159265358979323846 / N;
   std::vector<H8> T((size_t)S::COUNT * 64);
   const int H = S::H, half = 16 * S::NC;
+  // indices past N (the K padding of N = 48) are zero entries
   auto C = [&](int j, int c) {
+    if (j >= N) return 0.0;
     if (c < H) return std::cos(w * ((j * c) % N));
     if (c >= half && c - half < H) return -std::sin(w * ((j * (c - half)) % N));
     return 0.0;
   };
-  auto Fr = [&](int a, int b) { return std::cos(w * ((a * b) % N)); };
-  auto Fi = [&](int a, int b) { return -std::sin(w * ((a * b) % N)); };
-  auto Fn = [&](int a, int b) { return std::sin(w * ((a * b) % N)); };
+  auto Fr = [&](int a, int b) { return a < N && b < N ? std::cos(w * ((a * b) % N)) : 0.0; };
+  auto Fi = [&](int a, int b) { return a < N && b < N ? -std::sin(w * ((a * b) % N)) : 0.0; };
+  auto Fn = [&](int a, int b) { return a < N && b < N ? std::sin(w * ((a * b) % N)) : 0.0; };
   auto wk = [&](int k) { return (k == 0 || k == N / 2) ? 1.0 : 2.0; };
   // Dr^T / Dn^T: row j (output column of x), column k (spectral column; zero past H)
   auto DrT = [&](int j, int k) { return k < H ? wk(k) * std::cos(w * ((j * k) % N)) : 0.0; };
@@ -124,7 +126,7 @@ std::vector<H8> build_tables() {
 }  // namespace
 
 const H8* mfma_dft_table(int n, hipError_t* err) {
-  if (n != 32 && n != 64) {
+  if (n != 32 && n != 48 && n != 64) {
     *err = hipErrorInvalidValue;
     return nullptr;
   }
@@ -136,12 +138,12 @@ const H8* mfma_dft_table(int n, hipError_t* err) {
   }
   // one copy of the tables per device and block size, built on first use
   static std::mutex mu;
-  static std::vector<H8*> tabs[2];
+  static std::vector<H8*> tabs[3];
   std::lock_guard<std::mutex> lock(mu);
-  std::vector<H8*>& tab = tabs[n == 64];
+  std::vector<H8*>& tab = tabs[n == 32 ? 0 : n == 48 ? 1 : 2];
   if ((int)tab.size() <= dev) tab.resize(dev + 1, nullptr);
   if (!tab[dev]) {
-    std::vector<H8> h = n == 32 ? build_tables<32>() : build_tables<64>();
+    std::vector<H8> h = n == 32 ? build_tables<32>() : n == 48 ? build_tables<48>() : build_tables<64>();
     H8* d = nullptr;
     e = hipMalloc(&d, h.size() * sizeof(H8));
     if (e == hipSuccess) e = hipMemcpy(d, h.data(), h.size() * sizeof(H8), hipMemcpyHostToDevice);
@@ -158,7 +160,8 @@ const H8* mfma_dft_table(int n, hipError_t* err) {
 }
 
 hipError_t launch_dft_mfma(const void* in, void* out, int nb, int n, int inverse, float scale, hipStream_t s) {
-  if ((n != 32 && n != 64) || nb < 0) return hipErrorInvalidValue;
+  if ((n != 32 && n != 48 && n != 64) || nb < 0) return hipErrorInvalidValue;
+  if (nb > 0 && (!in || !out)) return hipErrorInvalidValue;
   if (!(scale > 0.0f) || std::ldexp(1.0f, std::ilogb(scale)) != scale) return hipErrorInvalidValue;
   if (nb == 0) return hipSuccess;
   hipError_t e;
@@ -170,6 +173,9 @@ hipError_t launch_dft_mfma(const void* in, void* out, int nb, int n, int inverse
   if (n == 32) {
     if (inverse) hipLaunchKernelGGL(k_dft_inv<32>, grid, block, 0, s, zin, static_cast<float*>(out), nb, scale, T);
     else hipLaunchKernelGGL(k_dft_fwd<32>, grid, block, 0, s, xin, static_cast<float2*>(out), nb, scale, T);
+  } else if (n == 48) {
+    if (inverse) hipLaunchKernelGGL(k_dft_inv<48>, grid, block, 0, s, zin, static_cast<float*>(out), nb, scale, T);
+    else hipLaunchKernelGGL(k_dft_fwd<48>, grid, block, 0, s, xin, static_cast<float2*>(out), nb, scale, T);
   } else {
     if (inverse) hipLaunchKernelGGL(k_dft_inv<64>, grid, block, 0, s, zin, static_cast<float*>(out), nb, scale, T);
     else hipLaunchKernelGGL(k_dft_fwd<64>, grid, block, 0, s, xin, static_cast<float2*>(out), nb, scale, T);

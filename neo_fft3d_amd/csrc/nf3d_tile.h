@@ -467,8 +467,8 @@ __device__ __forceinline__ void overlap_tile(const PlaneJob& J, uint8_t* dst, fl
 }
 
 // ------------------------------------------------------------------ kernel
-// MF: the 2D transforms as DFT-matrix products on the matrix cores (32x32
-// blocks; phases 2-3 and 5-6 become one forward and one inverse product chain
+// MF: the 2D transforms as DFT-matrix products on the matrix cores (32x32,
+// 48x48 and 64x64 blocks; phases 2-3 and 5-6 become one forward and one inverse product chain
 // per block, a wavefront per block), the rest unchanged.
 template <int BW, int BH, int NT, int MODE, int BT, int ONCHIP, bool MF = false>
 __global__ void __launch_bounds__(NT) k_tile(TileLaunch L_arg) {
@@ -524,7 +524,7 @@ __global__ void __launch_bounds__(NT) k_tile(TileLaunch L_arg) {
       // wavefront, highest blocks first: spectral block b overlaps only the
       // gather stage of blocks b and b+1, so a round writes over stages its own
       // round or earlier rounds have read (one barrier per round)
-      static_assert(BW == BH && (BW == 32 || BW == 64), "matrix-core DFT: 32x32 or 64x64 blocks");
+      static_assert(BW == BH && (BW == 32 || BW == 48 || BW == 64), "matrix-core DFT: 32x32, 48x48 or 64x64 blocks");
       static_assert(LSPEC >= LA && 2 * LA - LSPEC >= (BPW - 1) * (LSPEC - LA), "spectra overlap stages b, b+1 only");
       constexpr int NW = NT / 64, MB = (BPW + NW - 1) / NW;
       const int wave = __builtin_amdgcn_readfirstlane(t >> 6), lane = t & 63;
@@ -825,11 +825,11 @@ hipError_t launch_tile_modes(int mode, int bt, int onchip, const TileLaunch& L, 
   return hipGetLastError();
 }
 
-// mf: the matrix-core DFT variant (32x32 and 64x64 blocks)
+// mf: the matrix-core DFT variant (32x32, 48x48 and 64x64 blocks)
 template <int BW, int BH>
 hipError_t launch_tile_shape(int mode, int bt, int onchip, int mf, const TileLaunch& L, hipStream_t s) {
   if (!mf) return launch_tile_modes<BW, BH, false>(mode, bt, onchip, L, s);
-  if constexpr (BW == BH && (BW == 32 || BW == 64)) {
+  if constexpr (BW == BH && (BW == 32 || BW == 48 || BW == 64)) {
     for (int k = 0; k < L.njobs; k++)
       if (!L.job[k].dft_tab) return hipErrorInvalidValue;
     return launch_tile_modes<BW, BH, true>(mode, bt, onchip, L, s);

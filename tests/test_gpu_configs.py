@@ -10,7 +10,7 @@ workloads' own seeded generators at reduced sizes:
 
 * 4:2:0 8-bit moving textured rectangles, bt = 1 / 3 / 5, 48/16, sharpen 0.3
   (the scale workload; chroma planes centred on 128, src/buffer.cpp:133-135);
-* 4:2:0 10-bit, noise 12, 64/16 and 32/16 blocks, Kalman (kratio 2) and
+* 4:2:0 10-bit, noise 12, 64/16, 48/16 and 32/16 blocks, Kalman (kratio 2) and
   Wiener bt = 3 (the precision workload's shapes in the FFT form);
 * 4:4:4 16-bit, 32/10 (ow = nearest even to bw/3), bt = 5;
 * the Gray8 gratings workload at its full 640x360 size;
@@ -133,7 +133,7 @@ def test_scale_workload_yuv420p8_48_sharpen(torch_cuda, bt):
     compare_with_oracle(f"420p8 bt={bt}", clip, args, dc.outputs())
 
 
-@pytest.mark.parametrize("bw,ow", [(64, 16), (32, 16)])
+@pytest.mark.parametrize("bw,ow", [(64, 16), (48, 16), (32, 16)])
 @pytest.mark.parametrize("bt", [0, 3])
 def test_precision_workload_shapes_yuv420p10(torch_cuda, bw, ow, bt):
     from neo_fft3d_amd.synth import SEED_BASE

@@ -1,6 +1,6 @@
 """DFT-as-GEMM precision form (nf3d_dft2d_mfma) against the FP64 DFT.
 
-The 32x32 and 64x64 block transforms run as dense DFT-matrix products on the
+The 32x32, 48x48 and 64x64 block transforms run as dense DFT-matrix products on the
 matrix cores with FP16 hi/lo split operands and FP32 accumulation.  The closed form
 is numpy's float64 FFT (an FP64 DFT); the bar is a maximum error of 2e-6 of
 the largest spectral magnitude (forward) or of the largest sample (inverse),
@@ -48,7 +48,7 @@ def run(torch, lib, host_in, out_shape, out_dtype, inverse, scale, n=32):
     return d_out.cpu().numpy()
 
 
-@pytest.mark.parametrize("n", [32, 64])
+@pytest.mark.parametrize("n", [32, 48, 64])
 @pytest.mark.parametrize("maxv,scale", [(255.0, 1.0), (1023.0, 1.0), (65535.0, 0.25)])
 def test_forward_matches_fp64(env, maxv, scale, n):
     torch, lib = env
@@ -64,7 +64,7 @@ def test_forward_matches_fp64(env, maxv, scale, n):
     assert err.max() <= 2e-6 * ref
 
 
-@pytest.mark.parametrize("n", [32, 64])
+@pytest.mark.parametrize("n", [32, 48, 64])
 @pytest.mark.parametrize("maxv,scale", [(255.0, 1.0 / 32), (1023.0, 1.0 / 32), (65535.0, 1.0 / 4096)])
 def test_inverse_matches_fp64(env, maxv, scale, n):
     torch, lib = env
@@ -94,6 +94,14 @@ def test_round_trip_recovers_blocks(env):
 
 
 def test_rejects_unsupported_sizes(env):
+    """Argument checks on the host: no kernel is launched for any of these
+    (a block size without tables, a scale that is not a power of two, null
+    buffers with blocks to transform)."""
     torch, lib = env
+    buf = torch.zeros(64 * 64 * 2, dtype=torch.float32, device="cuda")
+    p = ctypes.c_void_p(buf.data_ptr())
+    assert lib.nf3d_dft2d_mfma(p, p, 1, 24, 0, ctypes.c_float(1.0), None) != 0
+    assert lib.nf3d_dft2d_mfma(p, p, 1, 32, 0, ctypes.c_float(0.3), None) != 0
     assert lib.nf3d_dft2d_mfma(None, None, 1, 48, 0, ctypes.c_float(1.0), None) != 0
-    assert lib.nf3d_dft2d_mfma(None, None, 1, 32, 0, ctypes.c_float(0.3), None) != 0
+    assert lib.nf3d_dft2d_mfma(p, None, 1, 32, 1, ctypes.c_float(1.0), None) != 0
+    torch.cuda.synchronize()

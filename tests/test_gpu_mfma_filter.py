@@ -1,11 +1,15 @@
 """The filter with its 2D transforms as DFT-matrix products on the matrix
 cores (FP16 hi/lo split operands, FP32 accumulation; `dft=1`, 32x32 blocks in
-the frame loop and the per-frame kernel, 64x64 blocks in the per-frame kernel)
-against the FP64 oracle, with the same bars as the FFT form: integer formats
-within +-1 LSB with >= 99.9% of pixels exact, float within 1e-4.  The 10-bit
-4:2:0 cases follow the precision-form workload (bt=3 Wiener and bt=0 Kalman at
-32/16 and 64/16) and report each plane's error against FP64 for both
-transform forms."""
+the frame loop and the per-frame kernel, 48x48 and 64x64 blocks in the
+per-frame kernel) against the FP64 oracle, with the same bars as the FFT
+form: integer formats within +-1 LSB with >= 99.9% of pixels exact, float
+within 1e-4.  The 10-bit 4:2:0 cases follow the precision-form workload (bt=3
+Wiener at 32/16, 48/16 and 64/16, bt=0 Kalman at 32/16 and 64/16; Kalman at
+48/16 is compared on a larger clip in test_gpu_configs.py, where one block
+whose reset FP32 and FP64 decide differently is a small share of a chroma
+plane) and report each plane's error against FP64 for both transform forms; the 8-bit 4:2:0 case is the
+1080p benchmark's filter settings (bt=3, 48/16, sharpen 0.3) on a small
+clip."""
 import numpy as np
 import pytest
 
@@ -27,6 +31,10 @@ MODES = [
     "sigma=3;bt=0;bw=64;bh=64;ow=16;oh=16;kratio=2",
     "sigma=3;bt=1;bw=64;bh=64;ow=16;oh=16;sharpen=0.5",
     "sigma=3;bt=5;bw=64;bh=64;ow=16;oh=16;dehalo=1",
+    "sigma=3;bt=3;bw=48;bh=48;ow=16;oh=16",
+    "sigma=3;bt=0;bw=48;bh=48;ow=16;oh=16;kratio=2",
+    "sigma=3;bt=1;bw=48;bh=48;ow=16;oh=16;sharpen=0.5",
+    "sigma=3;bt=5;bw=48;bh=48;ow=16;oh=16;dehalo=1",
 ]
 
 
@@ -48,8 +56,14 @@ def test_yuv_p16_mfma(product, oracle):
     check_clip(product, oracle, "yuv", 16, False, "sigma=3;bt=3;bw=32;bh=32;ow=16;oh=16;dft=1", exact=0.99)
 
 
+def test_yuv420p8_benchmark_settings_mfma(product, oracle):
+    check_clip(product, oracle, "yuv", 8, False, "sigma=2;bt=3;bw=48;bh=48;ow=16;oh=16;sharpen=0.3;dft=1", W=192,
+               H=128)
+
+
 @pytest.mark.parametrize("args", ["sigma=3;bt=3;bw=32;bh=32;ow=16;oh=16",
                                   "sigma=3;bt=0;kratio=2;bw=32;bh=32;ow=16;oh=16",
+                                  "sigma=3;bt=3;bw=48;bh=48;ow=16;oh=16",
                                   "sigma=3;bt=3;bw=64;bh=64;ow=16;oh=16",
                                   "sigma=3;bt=0;kratio=2;bw=64;bh=64;ow=16;oh=16"])
 def test_yuv420p10_precision_form_vs_fp64(product, oracle, args):
@@ -73,5 +87,5 @@ def test_yuv420p10_precision_form_vs_fp64(product, oracle, args):
 def test_rejects_other_block_shapes(product):
     from harness import FilterError
     clip = make_clip("gray", 8, False, width=160, height=128, nframes=3, seed=2)
-    with pytest.raises(FilterError, match="32x32 and 64x64"):
-        product.open("sigma=3;bw=48;bh=48;dft=1", clip)
+    with pytest.raises(FilterError, match="32x32, 48x48 and 64x64"):
+        product.open("sigma=3;bw=24;bh=24;dft=1", clip)
