@@ -16,9 +16,10 @@
 //                         during the MFMAs;
 //                         epilogues: bias + GELU, bias, bias + exp (PSD),
 //                         GELU backward, plain FP32 (weight gradients)
-//   k_fwd<EPI, KS>        inference layers: fragments loaded straight from
-//                         memory against the transposed weights, K split
-//                         over the workgroup's waves, coalesced epilogue
+//   k_fwdp / k_fwdw / k_fwd
+//                         inference layers (K <= 256, <= 1280, larger):
+//                         fragments loaded straight from memory against the
+//                         transposed weights, coalesced epilogue
 //   k_transpose           the transposed BF16 weights, after each update
 //   k_features_rows       log-power features of compact block spectra
 //   k_whittle             Whittle loss y + t e^-y and its gradient
@@ -660,8 +661,8 @@ __global__ void __launch_bounds__(NTW) k_fwdw(FwdArgs g) {
 // K <= 256: weights held in registers (k_fwdp), at most about 512
 // workgroups;
 // K <= 1280: eight K phases with every load in flight at once (k_fwdw);
-// otherwise K phases per layer, enough that each wave still has four or
-// more steps (k_fwd).
+// otherwise (64x64 blocks: 2,112 inputs) four K phases of two steps in
+// flight (k_fwd).
 template <int EPI>
 hipError_t fwd(const FwdArgs& g, hipStream_t s) {
   if (g.K <= 256) {
@@ -675,16 +676,7 @@ hipError_t fwd(const FwdArgs& g, hipStream_t s) {
     hipLaunchKernelGGL((k_fwdw<EPI>), dim3((g.N + 63) / 64, (g.M + 31) / 32), dim3(NTW), 0, s, g);
     return hipGetLastError();
   }
-  const int ns = (g.K + 31) / 32;
-  const int KS = ns >= 16 ? 4 : ns >= 8 ? 2 : 1;
-  const dim3 rows(1, (g.M + 31) / 32);
-  if (KS == 4) {
-    hipLaunchKernelGGL((k_fwd<EPI, 4>), dim3((g.N + 63) / 64, rows.y), dim3(NT), 0, s, g);
-  } else if (KS == 2) {
-    hipLaunchKernelGGL((k_fwd<EPI, 2>), dim3((g.N + 127) / 128, rows.y), dim3(NT), 0, s, g);
-  } else {
-    hipLaunchKernelGGL((k_fwd<EPI, 1>), dim3((g.N + 255) / 256, rows.y), dim3(NT), 0, s, g);
-  }
+  hipLaunchKernelGGL((k_fwd<EPI, 4>), dim3((g.N + 63) / 64, (g.M + 31) / 32), dim3(NT), 0, s, g);
   return hipGetLastError();
 }
 

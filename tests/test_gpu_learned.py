@@ -78,7 +78,11 @@ def predict(torch, m, xb, rows, exp_out=False, scale=1.0):
     return out.cpu().numpy().astype(np.float64)
 
 
-@pytest.mark.parametrize("dims,rows", [((D, NH, D), 300), ((40, 24, 56), 37), ((D, NH, D), 4096)])
+# the input widths pick the inference kernel: 1200 (48x48 blocks) the
+# eight-phase one, 40 and the hidden widths the register-weight one, 2112
+# (64x64 blocks) the four-phase one; the row counts its tile shapes
+@pytest.mark.parametrize("dims,rows", [((D, NH, D), 300), ((40, 24, 56), 37), ((D, NH, D), 4096),
+                                       ((64 * 33, 64, 128), 200)])
 def test_forward_matches_oracle(torch, dims, rows):
     n_in, nh, n_out = dims
     params = rand_params(n_in, nh, n_out, seed=rows)
