@@ -996,6 +996,10 @@ struct nf3d_engine {
     void* const* dst = nullptr;
     long long src_pitch = 0, dst_pitch = 0;
     std::vector<RunLaunch> launches;
+    // Wiener 3D: the clip-end frames of the range (btcur 1, the 2D gains)
+    // as one-frame launches of the frame-loop kernel in its 2D mode, instead
+    // of the per-frame path
+    std::vector<RunLaunch> edges;
   };
 
   RangePlan plan_range(int first, int count, const void* const* src, long long src_pitch, void* const* dst,
@@ -1069,6 +1073,25 @@ struct nf3d_engine {
       P.launches.push_back(R);
       m0 += nf;
     }
+    if (P.mode == nf3d::MODE_W3D) {
+      for (int n = first; n < first + count; n++) {
+        if (n >= P.a && n <= P.b) continue;
+        RunLaunch R{};
+        R.J = base_job();
+        R.J.sigma2 = ep.sigma * ep.sigma / norm;  // btcur = 1 (src/fft3d_engine.h's window rule)
+        R.J.seam = d_seam_run;  // the edge launches run apart from the others on the stream
+        if (learned) use_learned(R.J);
+        R.J.src_pitch = src_pitch;
+        R.J.dst_pitch = dst_pitch;
+        R.src[0] = static_cast<const uint8_t*>(src[n]);
+        R.dst[0] = static_cast<uint8_t*>(dst[n]);
+        R.m0 = n;
+        R.nframes = 1;
+        R.fpw = 1;
+        R.ntiles = noy * ntx;
+        P.edges.push_back(R);
+      }
+    }
     return P;
   }
 
@@ -1077,7 +1100,8 @@ struct nf3d_engine {
   void run_prologue(const RangePlan& P, hipStream_t s) {
     check(hipSetDevice(device), "hipSetDevice");
     const int stop = P.run ? P.a : P.first + P.count;
-    for (int n = P.first; n < stop; n++) run_one(n, P.src, P.src_pitch, P.dst, P.dst_pitch, s);
+    if (P.edges.empty())
+      for (int n = P.first; n < stop; n++) run_one(n, P.src, P.src_pitch, P.dst, P.dst_pitch, s);
     if (!P.run) return;
     if (ep.pfactor != 0 && !pattern_set && !learned) {
       const int pf = std::max(0, std::min(ep.pframe, frames - 1));
@@ -1133,7 +1157,8 @@ struct nf3d_engine {
         ring_tick[k % ep.bt] = ++tick;
       }
     }
-    for (int n = P.b + 1; n < P.first + P.count; n++) run_one(n, P.src, P.src_pitch, P.dst, P.dst_pitch, s);
+    if (P.edges.empty())
+      for (int n = P.b + 1; n < P.first + P.count; n++) run_one(n, P.src, P.src_pitch, P.dst, P.dst_pitch, s);
   }
 
   void get_frame_host(int n, const nf3d_frame_ref* frames_, int nframes, void* dst, int dst_stride) {
@@ -1248,21 +1273,22 @@ namespace {
 // Jobs that share a model and an output scale (the planes of a clip) are
 // predicted together: one feature pass per job into the first one's batch
 // buffer, then one set of GEMMs over all their blocks, each job's pattern
-// pointer set to its rows.
+// pointer set to its rows.  `mode` is the launch's kernel mode (the 2D
+// launches of a Wiener-3D range's clip-end frames scale by 1, not bt).
 void learned_predictions(const nf3d_run_item* items, const std::vector<nf3d_engine::RangePlan>& plans,
-                         const std::vector<int>& members, size_t c0, nf3d::RunBatch& B, hipStream_t s) {
+                         const std::vector<int>& members, size_t c0, nf3d::RunBatch& B, int mode, hipStream_t s) {
   std::vector<bool> done(B.njobs, false);
   for (int c = 0; c < B.njobs; c++) {
     const int k = members[c0 + c];
     nf3d_engine* e = items[k].engine;
     if (!e->learned || done[c]) continue;
-    const float mult = plans[k].mode == nf3d::MODE_W3D ? (float)e->ep.bt : 1.0f;
+    const float mult = mode == nf3d::MODE_W3D ? (float)e->ep.bt : 1.0f;
     const float scale = mult * e->learned_strength;
     std::vector<int> group;
     for (int c2 = c; c2 < B.njobs; c2++) {
       const int k2 = members[c0 + c2];
       nf3d_engine* e2 = items[k2].engine;
-      const float mult2 = plans[k2].mode == nf3d::MODE_W3D ? (float)e2->ep.bt : 1.0f;
+      const float mult2 = mode == nf3d::MODE_W3D ? (float)e2->ep.bt : 1.0f;
       if (!done[c2] && e2->learned == e->learned && mult2 * e2->learned_strength == scale) {
         group.push_back(c2);
         done[c2] = true;
@@ -1272,7 +1298,7 @@ void learned_predictions(const nf3d_run_item* items, const std::vector<nf3d_engi
       const int k2 = members[c0 + c2];
       nf3d_engine* e2 = items[k2].engine;
       const RunLaunch& R = B.job[c2];
-      if (plans[k2].mode == nf3d::MODE_W3D && e2->ep.bt >= 3) return e2->slot_ptr(R.m0 % e2->ep.bt);
+      if (mode == nf3d::MODE_W3D && e2->ep.bt >= 3) return e2->slot_ptr(R.m0 % e2->ep.bt);
       PlaneJob F = e2->base_job();
       F.src = static_cast<const uint8_t*>(plans[k2].src[R.m0]);
       F.src_pitch = plans[k2].src_pitch;
@@ -1365,12 +1391,57 @@ void run_batch(const nf3d_run_item* items, int n, hipStream_t s, bool timed, boo
           B.max_hc = std::max(B.max_hc, e->Hc);
           B.max_frames = std::max(B.max_frames, R.nframes);
         }
-        learned_predictions(items, plans, members, c0, B, s);
+        learned_predictions(items, plans, members, c0, B, plans[i].mode, s);
         for (int c = 0; c < B.njobs; c++) evs.push_back(items[members[c0 + c]].engine->timer(timed, B.job[c].nframes, s));
         // the seam pass follows on the same stream: run beside the next
         // launch it takes issue slots the latency-bound k_run needs more
         check(nf3d::launch_run(plans[i].mode, e0->ep.bt, plans[i].shp, e0->dft_mode, e0->ep.bw, e0->ep.bh, B, s),
               "frame-loop kernel");
+        check(nf3d::launch_seam_run(e0->ep.bw, e0->ep.bh, B, s), "seam kernel");
+        for (auto& ev : evs)
+          if (ev.second) check(hipEventRecord(ev.second, s), "hipEventRecord");
+      }
+    }
+  }
+  // the clip-end frames of Wiener-3D ranges: one-frame launches of the
+  // frame-loop kernel in its 2D mode, engines of the same shape, sharpen
+  // case and transform form together
+  std::vector<bool> edone(n, false);
+  for (int i = 0; i < n; i++) {
+    if (edone[i] || plans[i].edges.empty()) continue;
+    nf3d_engine* e0 = items[i].engine;
+    std::vector<int> group;
+    for (int k = i; k < n; k++) {
+      nf3d_engine* e = items[k].engine;
+      if (!edone[k] && !plans[k].edges.empty() && e->ep.bw == e0->ep.bw && e->ep.bh == e0->ep.bh &&
+          plans[k].shp == plans[i].shp && e->dft_mode == e0->dft_mode) {
+        group.push_back(k);
+        edone[k] = true;
+      }
+    }
+    size_t nq = 0;
+    for (int k : group) nq = std::max(nq, plans[k].edges.size());
+    for (size_t q = 0; q < nq; q++) {
+      std::vector<int> members;
+      for (int k : group)
+        if (q < plans[k].edges.size()) members.push_back(k);
+      for (size_t c0 = 0; c0 < members.size(); c0 += nf3d::RUN_MAX_JOBS) {
+        nf3d::RunBatch B{};
+        std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
+        for (size_t c = c0; c < members.size() && c < c0 + nf3d::RUN_MAX_JOBS; c++) {
+          const int k = members[c];
+          nf3d_engine* e = items[k].engine;
+          RunLaunch& R = B.job[B.njobs++];
+          R = plans[k].edges[q];
+          R.wg_begin = B.total_wg;
+          B.total_wg += R.ntiles;
+          B.max_hc = std::max(B.max_hc, e->Hc);
+          B.max_frames = 1;
+        }
+        learned_predictions(items, plans, members, c0, B, nf3d::MODE_W2D, s);
+        for (int c = 0; c < B.njobs; c++) evs.push_back(items[members[c0 + c]].engine->timer(timed, 1, s));
+        check(nf3d::launch_run(nf3d::MODE_W2D, 0, plans[i].shp, e0->dft_mode, e0->ep.bw, e0->ep.bh, B, s),
+              "frame-loop kernel (clip ends)");
         check(nf3d::launch_seam_run(e0->ep.bw, e0->ep.bh, B, s), "seam kernel");
         for (auto& ev : evs)
           if (ev.second) check(hipEventRecord(ev.second, s), "hipEventRecord");
