@@ -151,18 +151,19 @@ def gpu_run(args):
 
     # roofline of the filtering kernels (HIP events on each engine's stream):
     # the frame-loop launches (k_run + k_seam_run, 16 frames each) and the
-    # per-frame launches at the clip ends (k_tile + k_seam); the engines
-    # report the frames their timed launches filtered
+    # one-frame 2D launches of the clip ends; the engines report the frames
+    # their timed launches filtered
     pdims = [(W, H), (W // 2, H // 2), (W // 2, H // 2)]
     bytes_total = 0.0
     frames_timed = 0
+    edges = args.steps * (args.bt // 2 + (args.bt - 1) // 2) if args.bt >= 2 else 0
     for (pw, ph), (ms, n) in zip(pdims, prof):
-        # the 2 edge frames of each pass run the 2D path and are counted at
-        # the 3D bytes (<1% of the frames)
-        bytes_total += frame_plane_bytes(pw, ph, args.bt) * n
+        # the clip-end frames of each pass (bt // 2 + (bt - 1) // 2) run the
+        # 2D gains: pixels in and out, no spectra ring
+        bytes_total += frame_plane_bytes(pw, ph, args.bt) * (n - edges) + frame_plane_bytes(pw, ph, 1) * edges
         frames_timed += n
     # the planes share their launches: each engine's timed intervals are the
-    # same shared launches (plus its few per-frame edge launches)
+    # same shared launches
     ms_total = max(ms for ms, _ in prof)
     achieved = bytes_total / (ms_total * 1e-3) / 1e9 if ms_total > 0 else 0.0
     avg_us = ms_total * 1e3 / max(frames_timed, 1)
@@ -183,7 +184,7 @@ def gpu_run(args):
 
     # host-fed: the same frames from (pageable) host memory through the
     # pinned, chunked upload / filter / download pipeline (PCIe-inclusive)
-    host = host_fed(planes, args)
+    host = None if args.no_host_fed else host_fed(planes, args)
 
     # quality: denoised vs a noise-free render of the same clip
     idx = [F // 2 + k for k in range(4)]
@@ -806,6 +807,8 @@ def main():
     ap.add_argument("--frames", type=int, default=300)
     ap.add_argument("--bt", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-host-fed", action="store_true",
+                    help="skip the host-fed (PCIe) leg, e.g. for a kernel trace of the resident passes only")
     ap.add_argument("--no-precision-form", action="store_true",
                     help="skip the DFT-as-GEMM (MFMA, FP16 split) transform measurement")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-PSD leg")
