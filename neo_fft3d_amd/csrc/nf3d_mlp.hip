@@ -295,8 +295,18 @@ __global__ void __launch_bounds__(256) k_colsum1(const float* X, int rows, int c
 __global__ void k_colsum2(const float* part, int cols, int slices, float* out) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= cols) return;
+  // the slices added in order, their loads sixteen at a time in flight (a
+  // few hundred lanes in all: one load round trip per slice otherwise)
   float s = 0.0f;
-  for (int q = 0; q < slices; q++) s += part[q * cols + c];
+  int q = 0;
+  for (; q + 16 <= slices; q += 16) {
+    float v[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) v[u] = part[(q + u) * cols + c];
+#pragma unroll
+    for (int u = 0; u < 16; u++) s += v[u];
+  }
+  for (; q < slices; q++) s += part[q * cols + c];
   out[c] = s;
 }
 
