@@ -233,6 +233,15 @@ int nf3d_engine_region_psd(nf3d_engine* e, const void* src, int src_stride, int 
  * frames (frames per launch: nf3d_engine_set_tuning; 1 gives the per-frame
  * semantics). */
 int nf3d_engine_set_learned(nf3d_engine* e, nf3d_mlp* m, float strength, char* err, size_t errlen);
+/* Prediction period of the learned PSD in the frame-loop path (no reference
+ * counterpart; the reference's own pattern is estimated once, from frame
+ * pframe, for the whole clip): 0 (default) = one prediction per launch, as
+ * above; n > 0 = one per span of n frames, the spans starting at the
+ * multiples of n and at the first frame-loop frame of each call, every span
+ * filtered with the PSD of its first frame.  Launches are split at the span
+ * starts, so the result no longer depends on frames per launch.  Errors:
+ * n < 0. */
+int nf3d_engine_set_learned_period(nf3d_engine* e, int frames, char* err, size_t errlen);
 
 /* Spatially varying noise pattern (no reference counterpart; it generalises
  * the pattern estimate of src/helper.cpp:46-132): with pfactor != 0 the
@@ -264,8 +273,9 @@ int nf3d_engine_set_dft(nf3d_engine* e, int mode, char* err, size_t errlen);
  * named by NF3D_TUNE_TABLE or by nf3d_tune_load (which replaces it and returns
  * its line count); engines created afterwards take their entry, except that
  * attaching a learned PSD (whose prediction is made once per launch, so that
- * frames per launch is part of its result) restores the default 16 frames
- * per launch unless nf3d_engine_set_tuning set it.
+ * frames per launch is part of its result, unless a prediction period is
+ * set) restores the default 16 frames per launch unless
+ * nf3d_engine_set_tuning set it.
  * neo_fft3d_amd/autotune.py measures the choices and writes such a table. */
 int nf3d_tune_load(const char* path, char* err, size_t errlen);
 int nf3d_tune_lookup(int bw, int bh, int bits, int bt, int* out2);  /* 1 if found */

@@ -205,6 +205,10 @@ struct nf3d_engine {
   // pattern [nb][bh][outwidth] and the model's inference workspace
   nf3d_mlp* learned = nullptr;
   float learned_strength = 1.0f;
+  // frame-loop predictions: 0 = one per launch (from its first frame); n > 0
+  // = one per span of n frames (spans start at multiples of n and at the
+  // range's first frame-loop frame; launches never cross a span's start)
+  int learned_period = 0;
   float2* d_lspec = nullptr;
   void* d_lx = nullptr;
   float* d_lpat = nullptr;
@@ -996,6 +1000,8 @@ struct nf3d_engine {
     void* const* dst = nullptr;
     long long src_pitch = 0, dst_pitch = 0;
     std::vector<RunLaunch> launches;
+    // learned PSD: launch j predicts (else it keeps its span's PSD in d_lpat)
+    std::vector<char> predict;
     // Wiener 3D: the clip-end frames of the range (btcur 1, the 2D gains)
     // as one-frame launches of the frame-loop kernel in its 2D mode, instead
     // of the per-frame path
@@ -1047,31 +1053,48 @@ struct nf3d_engine {
     // buffers in turn
     const size_t seam_frame = (size_t)4 * Hc * Wc;
     if (!d_seam_run) d_seam_run = dev_alloc<float>(2 * kRunFrames * seam_frame);
-    const int total = P.b - P.a + 1;
-    const int nl = (total + run_frames - 1) / run_frames;
-    for (int j = 0, m0 = P.a; j < nl; j++) {
-      const int nf = total / nl + (j >= nl - total % nl ? 1 : 0);
-      RunLaunch R{};
-      R.J = base_job();
-      R.J.sigma2 = (float)(P.mode == nf3d::MODE_W3D ? ep.bt : 1) * ep.sigma * ep.sigma / norm;
-      R.J.seam = d_seam_run + (size_t)(j & 1) * kRunFrames * seam_frame;
-      // learned PSD: one per launch, from the launch's first frame (the
-      // engine writes it to d_lpat just before the launch)
-      if (learned) use_learned(R.J);
-      R.J.src_pitch = src_pitch;
-      R.J.dst_pitch = dst_pitch;
-      for (int i = 0; i < nf; i++) {
-        R.src[i] = static_cast<const uint8_t*>(src[m0 + i + P.to]);
-        R.dst[i] = static_cast<uint8_t*>(dst[m0 + i]);
+    // the spans of one learned-PSD prediction each (the whole range without
+    // a prediction period), each in launches of equal length
+    std::vector<std::pair<int, int>> spans;
+    if (learned && learned_period > 0) {
+      for (int a = P.a; a <= P.b;) {
+        const int b = std::min(P.b, (a / learned_period + 1) * learned_period - 1);
+        spans.push_back({a, b});
+        a = b + 1;
       }
-      R.ring = d_ring;
-      R.spec_elems = (long long)spec_elems;
-      R.m0 = m0;
-      R.nframes = nf;
-      R.fpw = (P.mode == nf3d::MODE_W3D || P.mode == nf3d::MODE_KALMAN) ? nf : std::min(run_fpw, nf);
-      R.ntiles = noy * ntx;
-      P.launches.push_back(R);
-      m0 += nf;
+    } else {
+      spans.push_back({P.a, P.b});
+    }
+    int j = 0;
+    for (const auto& sp : spans) {
+      const int total = sp.second - sp.first + 1;
+      const int nl = (total + run_frames - 1) / run_frames;
+      for (int jj = 0, m0 = sp.first; jj < nl; jj++, j++) {
+        const int nf = total / nl + (jj >= nl - total % nl ? 1 : 0);
+        RunLaunch R{};
+        R.J = base_job();
+        R.J.sigma2 = (float)(P.mode == nf3d::MODE_W3D ? ep.bt : 1) * ep.sigma * ep.sigma / norm;
+        R.J.seam = d_seam_run + (size_t)(j & 1) * kRunFrames * seam_frame;
+        // learned PSD in d_lpat: predicted from the launch's first frame just
+        // before the launch, or, with a prediction period, before the first
+        // launch of each span only
+        if (learned) use_learned(R.J);
+        R.J.src_pitch = src_pitch;
+        R.J.dst_pitch = dst_pitch;
+        for (int i = 0; i < nf; i++) {
+          R.src[i] = static_cast<const uint8_t*>(src[m0 + i + P.to]);
+          R.dst[i] = static_cast<uint8_t*>(dst[m0 + i]);
+        }
+        R.ring = d_ring;
+        R.spec_elems = (long long)spec_elems;
+        R.m0 = m0;
+        R.nframes = nf;
+        R.fpw = (P.mode == nf3d::MODE_W3D || P.mode == nf3d::MODE_KALMAN) ? nf : std::min(run_fpw, nf);
+        R.ntiles = noy * ntx;
+        P.launches.push_back(R);
+        P.predict.push_back(learned_period <= 0 || jj == 0);
+        m0 += nf;
+      }
     }
     if (P.mode == nf3d::MODE_W3D) {
       for (int n = first; n < first + count; n++) {
@@ -1273,11 +1296,19 @@ namespace {
 // Jobs that share a model and an output scale (the planes of a clip) are
 // predicted together: one feature pass per job into the first one's batch
 // buffer, then one set of GEMMs over all their blocks, each job's pattern
-// pointer set to its rows.  `mode` is the launch's kernel mode (the 2D
-// launches of a Wiener-3D range's clip-end frames scale by 1, not bt).
+// pointer set to its rows -- or, for an engine with a prediction period, its
+// rows copied to its own d_lpat, which its later launches of the span reuse.
+// `mode` is the launch's kernel mode (the 2D launches of a Wiener-3D range's
+// clip-end frames scale by 1, not bt); `l` the launch's index in the jobs'
+// plans (SIZE_MAX for the clip-end launches, which all predict).
 void learned_predictions(const nf3d_run_item* items, const std::vector<nf3d_engine::RangePlan>& plans,
-                         const std::vector<int>& members, size_t c0, nf3d::RunBatch& B, int mode, hipStream_t s) {
+                         const std::vector<int>& members, size_t c0, nf3d::RunBatch& B, int mode, size_t l,
+                         hipStream_t s) {
   std::vector<bool> done(B.njobs, false);
+  for (int c = 0; c < B.njobs; c++) {
+    const int k = members[c0 + c];
+    if (l != SIZE_MAX && !plans[k].predict[l]) done[c] = true;
+  }
   for (int c = 0; c < B.njobs; c++) {
     const int k = members[c0 + c];
     nf3d_engine* e = items[k].engine;
@@ -1339,14 +1370,25 @@ void learned_predictions(const nf3d_run_item* items, const std::vector<nf3d_engi
       nf3d_engine* e2 = items[members[c0 + c2]].engine;
       specs[nseg] = spectra(c2);
       seg_rows[nseg++] = e2->nb;
-      B.job[c2].J.pattern2d = e->d_bp + row * n;
-      B.job[c2].J.pattern3d = e->d_bp + row * n;
+      if (l == SIZE_MAX || e2->learned_period <= 0) {
+        B.job[c2].J.pattern2d = e->d_bp + row * n;
+        B.job[c2].J.pattern3d = e->d_bp + row * n;
+      }
       row += e2->nb;
     }
     if (nf3d_mlp_features_batch(e->learned, specs, seg_rows, nseg, e->d_bx, s)) fail("learned PSD: feature kernel");
     char msg[256] = {0};
     if (nf3d_mlp_predict(e->learned, e->d_bx, (int)rows, e->d_bp, 1, scale, e->d_bwork, s, msg, sizeof msg))
       fail(std::string("learned PSD: ") + msg);
+    row = 0;
+    for (int c2 : group) {
+      nf3d_engine* e2 = items[members[c0 + c2]].engine;
+      if (l != SIZE_MAX && e2->learned_period > 0)
+        check(hipMemcpyAsync(e2->d_lpat, e->d_bp + row * n, sizeof(float) * (size_t)(e2->nb * n),
+                             hipMemcpyDeviceToDevice, s),
+              "learned PSD: copy");
+      row += e2->nb;
+    }
   }
 }
 
@@ -1391,7 +1433,7 @@ void run_batch(const nf3d_run_item* items, int n, hipStream_t s, bool timed, boo
           B.max_hc = std::max(B.max_hc, e->Hc);
           B.max_frames = std::max(B.max_frames, R.nframes);
         }
-        learned_predictions(items, plans, members, c0, B, plans[i].mode, s);
+        learned_predictions(items, plans, members, c0, B, plans[i].mode, l, s);
         for (int c = 0; c < B.njobs; c++) evs.push_back(items[members[c0 + c]].engine->timer(timed, B.job[c].nframes, s));
         // the seam pass follows on the same stream: run beside the next
         // launch it takes issue slots the latency-bound k_run needs more
@@ -1438,7 +1480,7 @@ void run_batch(const nf3d_run_item* items, int n, hipStream_t s, bool timed, boo
           B.max_hc = std::max(B.max_hc, e->Hc);
           B.max_frames = 1;
         }
-        learned_predictions(items, plans, members, c0, B, nf3d::MODE_W2D, s);
+        learned_predictions(items, plans, members, c0, B, nf3d::MODE_W2D, SIZE_MAX, s);
         for (int c = 0; c < B.njobs; c++) evs.push_back(items[members[c0 + c]].engine->timer(timed, 1, s));
         check(nf3d::launch_run(nf3d::MODE_W2D, 0, plans[i].shp, e0->dft_mode, e0->ep.bw, e0->ep.bh, B, s),
               "frame-loop kernel (clip ends)");
@@ -2003,6 +2045,18 @@ int nf3d_engine_set_learned(nf3d_engine* e, nf3d_mlp* m, float strength, char* e
       e->run_frames = nf3d_engine::kRunFrames;
       e->run_frames_from_table = false;
     }
+  } catch (const Failure& f) {
+    copy_err(err, errlen, f.msg);
+    return -1;
+  }
+  return 0;
+}
+
+int nf3d_engine_set_learned_period(nf3d_engine* e, int frames, char* err, size_t errlen) {
+  std::lock_guard<std::mutex> lock(e->mu);
+  try {
+    if (frames < 0) fail("learned PSD: the prediction period must be 0 (one per launch) or a frame count");
+    e->learned_period = frames;
   } catch (const Failure& f) {
     copy_err(err, errlen, f.msg);
     return -1;

@@ -67,6 +67,7 @@ def _bind():
     L.nf3d_mlp_train_step.argtypes = [vp, vp, vp, ip, fp, ctypes.POINTER(ctypes.c_double), vp, cp, sz]
     L.nf3d_engine_spectra.argtypes = [vp, vp, ip, vp, vp, cp, sz]
     L.nf3d_engine_set_learned.argtypes = [vp, vp, fp, cp, sz]
+    L.nf3d_engine_set_learned_period.argtypes = [vp, ip, cp, sz]
     L._mlp_bound = True
     return L
 
@@ -205,10 +206,15 @@ def region_labels(engine: Engine, src_ptr: int, stride: int, rx: int, ry: int, o
     engine.sync()
 
 
-def set_learned(engine: Engine, mlp: Mlp | None, strength: float = 1.0):
+def set_learned(engine: Engine, mlp: Mlp | None, strength: float = 1.0, period: int | None = None):
+    """Attach (or with None detach) a learned PSD; `period` (frames) sets
+    how often the frame-loop path predicts it: 0 once per launch, n > 0 once
+    per span of n frames (None leaves the engine's setting)."""
     err = ctypes.create_string_buffer(512)
     if _bind().nf3d_engine_set_learned(engine.h, mlp.h if mlp is not None else None, ctypes.c_float(strength),
                                        err, 512) != 0:
+        raise Nf3dError(err.value.decode())
+    if period is not None and _bind().nf3d_engine_set_learned_period(engine.h, int(period), err, 512) != 0:
         raise Nf3dError(err.value.decode())
 
 

@@ -302,10 +302,12 @@ def test_learned_psd_gains_psnr(torch):
     assert min(gains) > 0.0
 
 
-def psd_source_frame(n, bt, F, per_launch):
+def psd_source_frame(n, bt, F, per_launch, period=0):
     """The frame whose PSD the engine uses for frame n of an in-order device
     range 0..F-1: frames of the frame-loop range take their launch's first
-    frame (launches split as the engine splits them), the others their own."""
+    frame (launches split as the engine splits them) or, with a prediction
+    period, their span's first frame (spans start at multiples of the period
+    and at the range's first frame); the others their own."""
     if bt >= 2:
         lo, hi = bt // 2, F - 1 - (bt - 1) // 2
     elif bt == 0:
@@ -314,6 +316,8 @@ def psd_source_frame(n, bt, F, per_launch):
         lo, hi = 0, F - 1
     if n < lo or n > hi:
         return n
+    if period > 0:
+        return max(lo, n // period * period)
     total = hi - lo + 1
     nl = (total + per_launch - 1) // per_launch
     m0 = lo
@@ -325,14 +329,17 @@ def psd_source_frame(n, bt, F, per_launch):
     raise AssertionError("frame outside the launches")
 
 
-@pytest.mark.parametrize("family,bits,is_float,bt,per_launch", [
-    ("rgb", 32, True, 3, 1), ("gray", 8, False, 1, 1), ("gray", 8, False, 0, 1), ("rgb", 32, True, 5, 1),
-    ("rgb", 32, True, 3, 16), ("gray", 8, False, 1, 4), ("gray", 8, False, 0, 3)])
-def test_learned_path_matches_oracle(torch, family, bits, is_float, bt, per_launch):
+@pytest.mark.parametrize("family,bits,is_float,bt,per_launch,period", [
+    ("rgb", 32, True, 3, 1, 0), ("gray", 8, False, 1, 1, 0), ("gray", 8, False, 0, 1, 0), ("rgb", 32, True, 5, 1, 0),
+    ("rgb", 32, True, 3, 16, 0), ("gray", 8, False, 1, 4, 0), ("gray", 8, False, 0, 3, 0),
+    ("rgb", 32, True, 3, 2, 4), ("gray", 8, False, 1, 3, 4), ("gray", 8, False, 0, 16, 3)])
+def test_learned_path_matches_oracle(torch, family, bits, is_float, bt, per_launch, period):
     """The engine's learned-PSD filtering against the FP64 oracle given the
     same per-block PSDs (the model's exp(log-PSD) of the frame the engine
     predicts from): per-block pattern indexing in both kernels, the exp
-    epilogue, the btcur scaling and the one-PSD-per-launch rule."""
+    epilogue, the btcur scaling, the one-PSD-per-launch rule and, with a
+    prediction period, the one-PSD-per-span rule (launches of a span after
+    its first keep its PSD)."""
     import ctypes
     from harness import Clip, compare, oracle_runner
     from neo_fft3d_amd import Engine, default_params, make_format
@@ -390,7 +397,7 @@ def test_learned_path_matches_oracle(torch, family, bits, is_float, bt, per_laun
     from neo_fft3d_amd.learned import set_learned
     gpu = []
     for p in range(nplanes):
-        set_learned(engines[p], m, 1.0)
+        set_learned(engines[p], m, 1.0, period=period)
         set_tuning(engines[p], per_launch, 1)
         dst = torch.zeros_like(srcs[p])
         torch.cuda.synchronize()  # the engine's stream does not wait for torch's
@@ -404,13 +411,14 @@ def test_learned_path_matches_oracle(torch, family, bits, is_float, bt, per_laun
     orc.lib.nf3d_oracle_set_block_psd.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
     h = orc.open(args, clip)
     for n in range(F):
-        src_n = psd_source_frame(n, bt, F, per_launch)
+        src_n = psd_source_frame(n, bt, F, per_launch, period)
         for p in range(nplanes):
             assert orc.lib.nf3d_oracle_set_block_psd(h.h, p, psds[(p, src_n)].ctypes.data) == 0
         want = h.get(n)
         got = [gpu[p][n] for p in range(nplanes)]
         for p, (mx, fe, rms) in enumerate(compare(got, want, is_float)):
-            print(f"learned {family}{bits} bt={bt} per_launch={per_launch} n={n} (PSD of frame {src_n}) p={p}: "
+            print(f"learned {family}{bits} bt={bt} per_launch={per_launch} period={period} n={n} "
+                  f"(PSD of frame {src_n}) p={p}: "
                   f"max {mx:.3g} exact {fe:.5f} rms {rms:.3g} "
                   f"(log-PSD spread {spread:.2f})", flush=True)
             if is_float:
@@ -421,11 +429,14 @@ def test_learned_path_matches_oracle(torch, family, bits, is_float, bt, per_laun
         e.close()
 
 
-def test_graph_replays_with_learned_psd(torch):
+@pytest.mark.parametrize("period", [0, 4])
+def test_graph_replays_with_learned_psd(torch, period):
     """A batched call over engines with the learned PSD, captured as a HIP
     graph: every replay (with eager work between replays) equals the eager
     call bit for bit -- the per-launch prediction (analysis, features, GEMMs)
-    is part of the graph."""
+    is part of the graph.  With a prediction period the planes' shared
+    prediction is copied to each engine's own PSD, which the span's later
+    launches reuse: the batched call equals the planes run one by one."""
     from neo_fft3d_amd import Engine, Graph, default_params, make_format, run_device_batch
     from neo_fft3d_amd.learned import Mlp, noisy_clip, set_learned
     F, W, H = 10, 160, 112
@@ -439,7 +450,11 @@ def test_graph_replays_with_learned_psd(torch):
     fmt = make_format("rgb", 32, True)
     engs = [Engine(default_params(6.0, bt=3, bw=48, bh=48, ow=16, oh=16), fmt, W, H, F, p) for p in range(3)]
     for e in engs:
-        set_learned(e, m, 1.0)
+        set_learned(e, m, 1.0, period=period)
+    if period:
+        from neo_fft3d_amd.autotune import set_tuning
+        for e in engs:
+            set_tuning(e, 2, 1)  # spans of 4 frames in launches of 2: every other launch reuses
     src = [torch.from_numpy(np.ascontiguousarray(noisy[p])).cuda() for p in range(3)]
     dst = [torch.zeros_like(x) for x in src]
     st = W * 4
@@ -451,6 +466,16 @@ def test_graph_replays_with_learned_psd(torch):
         run_device_batch(items, stream.cuda_stream)
     stream.synchronize()
     want = [d.cpu().numpy().copy() for d in dst]
+    if period:
+        for p, e in enumerate(engs):
+            one = torch.zeros_like(src[p])
+            with torch.cuda.stream(stream):
+                run_device_batch([(e, 0, F, items[p][3], st, [one[k].data_ptr() for k in range(F)], st)],
+                                 stream.cuda_stream)
+            stream.synchronize()
+            print(f"period {period} plane {p} alone vs batched: max diff "
+                  f"{np.abs(one.cpu().numpy() - want[p]).max():.3g}", flush=True)
+            assert np.array_equal(one.cpu().numpy(), want[p]), p
     g = Graph(items, stream.cuda_stream)
     for r in range(3):
         with torch.cuda.stream(stream):
