@@ -442,26 +442,29 @@ def multi_clip(args):
         for e in c["es"]:
             e.close()
     graph = serving_graphs(clips, F, models)
+    graph_ring = serving_graphs(clips, F, models, ring=True)
     return {"workload": "16 clips: 4x 854x480 P8, 4x 1280x720 P8, 4x 1920x1080 P10, 4x 3840x2160 YUV444P16; "
                         "bt {0,1,3,5}, bw {32,48,64}, ow 10/16/22, sigma U[1.5,4]",
             "clips": [{k: c[k] for k in ("w", "h", "bits", "bt", "bw", "sigma", "learned")} for c in clips],
             "frame_steps": steps, "fps_aggregate": round(len(clips) * steps / total, 1),
             "step_latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
             "step_latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
-            "graph_replay": graph,
+            "graph_replay": graph, "graph_replay_ring": graph_ring,
             "learned_psd": f"on for {sum(c['learned'] for c in clips)} clips, {len(models)} networks "
                            f"(setup incl. data and training {t_train:.1f} s)"}
 
 
-def serving_graphs(clips, F, models):
+def serving_graphs(clips, F, models, ring=False):
     """The same 16 clips served as a stream: frame j of each plane arrives in
     staging slot j % S (S = 15, a multiple of every bt) and is filtered into
     output slot j % S, so a frame step's work repeats every S steps.  After
     three eager steps, the batched call of each phase k mod S is captured as
-    one HIP graph (nf3d_run_device_batch_graph) and replayed once to advance
-    the spectra ring; the timed steps are the frame's arrival (a device copy
-    per plane) plus one graph launch per phase.  Fresh engines, same
-    parameters and networks."""
+    one HIP graph (nf3d_run_device_batch_graph; with ring=True
+    nf3d_run_device_batch_graph_ring, whose steps read the spectra the
+    previous step left in the rings) and replayed once to advance the
+    spectra ring; the timed steps are the frame's arrival (a device copy per
+    plane, issued as one multi-tensor copy) plus one graph launch per phase.
+    Fresh engines, same parameters and networks."""
     import numpy as np
     import torch
 
@@ -483,9 +486,8 @@ def serving_graphs(clips, F, models):
         gcl.append(dict(src=c["src"], stg=stg, out=out, es=es))
 
     def arrive(j):
-        for g in gcl:
-            for p in range(3):
-                g["stg"][p][j % S].copy_(g["src"][p][j])
+        torch._foreach_copy_([g["stg"][p][j % S] for g in gcl for p in range(3)],
+                             [g["src"][p][j] for g in gcl for p in range(3)])
 
     def items(k):
         res = []
@@ -508,7 +510,7 @@ def serving_graphs(clips, F, models):
             if k < warm:
                 run_device_batch(items(k), sp)
             else:
-                graphs[k % S] = Graph(items(k), sp)
+                graphs[k % S] = Graph(items(k), sp, ring=ring)
                 graphs[k % S].launch(sp)
             stream.synchronize()
         t0 = time.perf_counter()

@@ -144,6 +144,17 @@ int nf3d_run_host_batch(const nf3d_run_item* items, int n, char* err, size_t err
 typedef struct nf3d_graph nf3d_graph;
 int nf3d_run_device_batch_graph(const nf3d_run_item* items, int n, void* stream, nf3d_graph** graph, char* err,
                                 size_t errlen);
+/* The same capture bound to the engines' spectra rings as they stand (no
+ * reference counterpart; for a stream of frame steps replayed in order): the
+ * captured work reads the Wiener-3D spectra the rings hold instead of
+ * transforming those frames again.  nf3d_graph_launch then replays it only
+ * if every engine's ring holds the frames it held at capture, shifted by one
+ * whole number of frames per engine (a multiple of bt: frame f sits in slot
+ * f mod bt), and returns -1 without launching otherwise; the caller puts the
+ * shifted frames at the captured frame pointers (staging slots reused
+ * periodically, as a serving loop does). */
+int nf3d_run_device_batch_graph_ring(const nf3d_run_item* items, int n, void* stream, nf3d_graph** graph,
+                                     char* err, size_t errlen);
 int nf3d_graph_launch(nf3d_graph* graph, void* stream);
 void nf3d_graph_destroy(nf3d_graph* graph);
 

@@ -91,6 +91,7 @@ def lib() -> ctypes.CDLL:
         L.nf3d_run_host_batch.argtypes = [ctypes.POINTER(RunItem), ip, ctypes.c_char_p, sz]
         L.nf3d_run_device_batch_graph.argtypes = [ctypes.POINTER(RunItem), ip, vp, ctypes.POINTER(vp),
                                                   ctypes.c_char_p, sz]
+        L.nf3d_run_device_batch_graph_ring.argtypes = L.nf3d_run_device_batch_graph.argtypes
         L.nf3d_graph_launch.argtypes = [vp, vp]
         L.nf3d_graph_destroy.argtypes = [vp]
         L.nf3d_graph_destroy.restype = None
@@ -245,20 +246,24 @@ def run_host_batch(items):
 
 
 class Graph:
-    """A captured nf3d_run_device_batch call (nf3d_run_device_batch_graph)."""
+    """A captured nf3d_run_device_batch call (nf3d_run_device_batch_graph, or
+    with ring=True nf3d_run_device_batch_graph_ring: bound to the spectra
+    rings as they stand, replayed only while they hold the same frames
+    shifted by whole frame steps)."""
 
-    def __init__(self, items, stream: int):
+    def __init__(self, items, stream: int, ring: bool = False):
         arr, keep = _run_items(items)
         g = ctypes.c_void_p()
         err = ctypes.create_string_buffer(512)
-        if lib().nf3d_run_device_batch_graph(arr, len(items), ctypes.c_void_p(stream), ctypes.byref(g),
-                                             err, 512) != 0:
+        cap = lib().nf3d_run_device_batch_graph_ring if ring else lib().nf3d_run_device_batch_graph
+        if cap(arr, len(items), ctypes.c_void_p(stream), ctypes.byref(g), err, 512) != 0:
             raise Nf3dError(err.value.decode())
         self.g = g
 
     def launch(self, stream: int):
         if lib().nf3d_graph_launch(self.g, ctypes.c_void_p(stream)) != 0:
-            raise Nf3dError("graph launch failed")
+            raise Nf3dError("graph launch failed (a ring-bound graph needs the rings to hold the frames it was "
+                            "captured against, shifted by whole frame steps)")
 
     def close(self):
         if getattr(self, "g", None):

@@ -1626,6 +1626,12 @@ struct nf3d_graph {
   std::vector<nf3d_engine*> engines;
   std::vector<std::vector<int>> ring_after;  // ring_frame of each engine after one replay
   std::vector<std::vector<char>> by_run_after;
+  // captured against the rings as they stood (nf3d_run_device_batch_graph_ring):
+  // what each ring held then; a replay needs the same frames, shifted by a
+  // whole number of frames per engine
+  bool ring_bound = false;
+  std::vector<std::vector<int>> ring_before;
+  std::vector<std::vector<char>> by_run_before;
 };
 
 int nf3d_run_host_batch(const nf3d_run_item* items, int n, char* err, size_t errlen) {
@@ -1862,8 +1868,9 @@ int nf3d_run_device_batch(const nf3d_run_item* items, int n, void* stream, char*
   return 0;
 }
 
-int nf3d_run_device_batch_graph(const nf3d_run_item* items, int n, void* stream, nf3d_graph** graph, char* err,
-                                size_t errlen) {
+namespace {
+int capture_batch(const nf3d_run_item* items, int n, void* stream, nf3d_graph** graph, bool keep_ring, char* err,
+                  size_t errlen) {
   *graph = nullptr;
   hipStream_t s = static_cast<hipStream_t>(stream);
   bool capturing = false;
@@ -1899,23 +1906,38 @@ int nf3d_run_device_batch_graph(const nf3d_run_item* items, int n, void* stream,
     } restore{items, prof};
     // a graph must not depend on what the spectra ring holds when it is
     // replayed: forget its contents, so the captured work transforms every
-    // frame it reads
+    // frame it reads -- unless it is bound to the rings as they stand (then
+    // the captured work reads the frames they hold, and a replay checks them)
+    std::vector<std::vector<int>> before(n);
+    std::vector<std::vector<char>> by_before(n);
     for (int i = 0; i < n; i++) {
       nf3d_engine* e = items[i].engine;
-      std::fill(e->ring_frame.begin(), e->ring_frame.end(), -1);
+      before[i] = e->ring_frame;
+      by_before[i] = e->ring_by_run;
+      if (!keep_ring) std::fill(e->ring_frame.begin(), e->ring_frame.end(), -1);
     }
     check(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
     capturing = true;
     run_batch(items, n, s, false, true);
     auto* g = new nf3d_graph;
     capturing = false;
+    g->ring_bound = keep_ring;
     // what a replay leaves in each ring; nothing has run yet
     for (int i = 0; i < n; i++) {
       nf3d_engine* e = items[i].engine;
       g->engines.push_back(e);
       g->ring_after.push_back(e->ring_frame);
       g->by_run_after.push_back(e->ring_by_run);
-      std::fill(e->ring_frame.begin(), e->ring_frame.end(), -1);
+      if (keep_ring) {
+        e->ring_frame = before[i];
+        e->ring_by_run = by_before[i];
+      } else {
+        std::fill(e->ring_frame.begin(), e->ring_frame.end(), -1);
+      }
+    }
+    if (keep_ring) {
+      g->ring_before = std::move(before);
+      g->by_run_before = std::move(by_before);
     }
     hipError_t e1 = hipStreamEndCapture(s, &g->graph);
     if (e1 != hipSuccess) {
@@ -1940,6 +1962,41 @@ int nf3d_run_device_batch_graph(const nf3d_run_item* items, int n, void* stream,
   return 0;
 }
 
+// The frames by which each engine's ring is ahead of what a ring-bound graph
+// was captured against (every slot the capture saw shifted alike, the
+// slots' kernel-written flags equal), or false.
+bool ring_shift(const nf3d_graph* g, std::vector<int>& shift) {
+  shift.assign(g->engines.size(), 0);
+  for (size_t i = 0; i < g->engines.size(); i++) {
+    const nf3d_engine* e = g->engines[i];
+    const std::vector<int>& was = g->ring_before[i];
+    if (e->ring_frame.size() != was.size()) return false;
+    bool have = false;
+    for (size_t k = 0; k < was.size(); k++) {
+      if (was[k] < 0) continue;
+      const int d = e->ring_frame[k] - was[k];
+      if (e->ring_frame[k] < 0 || e->ring_by_run[k] != g->by_run_before[i][k] || (have && d != shift[i]))
+        return false;
+      shift[i] = d;
+      have = true;
+    }
+    // the frame loop's slot of frame f is f mod bt
+    if (have && e->ep.bt >= 2 && shift[i] % e->ep.bt != 0) return false;
+  }
+  return true;
+}
+}  // namespace
+
+int nf3d_run_device_batch_graph(const nf3d_run_item* items, int n, void* stream, nf3d_graph** graph, char* err,
+                                size_t errlen) {
+  return capture_batch(items, n, stream, graph, false, err, errlen);
+}
+
+int nf3d_run_device_batch_graph_ring(const nf3d_run_item* items, int n, void* stream, nf3d_graph** graph,
+                                     char* err, size_t errlen) {
+  return capture_batch(items, n, stream, graph, true, err, errlen);
+}
+
 int nf3d_graph_launch(nf3d_graph* g, void* stream) {
   if (!g || !g->exec) return -1;
   std::vector<nf3d_engine*> es = g->engines;
@@ -1947,6 +2004,8 @@ int nf3d_graph_launch(nf3d_graph* g, void* stream) {
   std::vector<std::unique_lock<std::mutex>> locks;
   for (nf3d_engine* e : es) locks.emplace_back(e->mu);
   hipStream_t s = static_cast<hipStream_t>(stream);
+  std::vector<int> shift(g->engines.size(), 0);
+  if (g->ring_bound && !ring_shift(g, shift)) return -1;
   try {
     for (nf3d_engine* e : g->engines) e->order_before(s);
     check(hipGraphLaunch(g->exec, s), "hipGraphLaunch");
@@ -1954,10 +2013,14 @@ int nf3d_graph_launch(nf3d_graph* g, void* stream) {
   } catch (const Failure&) {
     return -1;
   }
-  // the replay leaves the frames recorded at capture in the rings
+  // the replay leaves the frames recorded at capture in the rings (moved on
+  // by the rings' shift, for a ring-bound graph)
   for (size_t i = 0; i < g->engines.size(); i++) {
-    g->engines[i]->ring_frame = g->ring_after[i];
-    g->engines[i]->ring_by_run = g->by_run_after[i];
+    nf3d_engine* e = g->engines[i];
+    e->ring_frame = g->ring_after[i];
+    for (int& f : e->ring_frame)
+      if (f >= 0) f += shift[i];
+    e->ring_by_run = g->by_run_after[i];
   }
   return 0;
 }

@@ -256,13 +256,16 @@ def test_mixed_set_serving_graphs_bit_exact(torch_cuda):
     call per step, (b) per-clip calls per step, (c) after two eager steps, one
     HIP graph per phase k mod S captured from the batched call of that step
     (each replayed once at capture to advance the ring), then replayed for
-    the following periods.  Every output frame of (b) and (c) equals (a) bit
-    for bit; the learned clips use networks whose PSD varies per block."""
+    the following periods, (d) as (c) with graphs bound to the spectra rings
+    (nf3d_run_device_batch_graph_ring: the captured steps read the frames the
+    previous step left in the rings instead of transforming them again).
+    Every output frame of (b), (c) and (d) equals (a) bit for bit; the
+    learned clips use networks whose PSD varies per block."""
     from neo_fft3d_amd import Graph, run_device_batch
     from neo_fft3d_amd.synth import SEED_BASE
     torch = torch_cuda
     F = 40
-    sets = {"batch": [], "single": [], "graph": []}
+    sets = {"batch": [], "single": [], "graph": [], "ring": []}
     models = []
     for k, (w, h, ss, bits, bw, ow, bt, sigma, learned) in enumerate(MIX):
         scale = 2.0 ** (bits - 8)
@@ -275,7 +278,7 @@ def test_mixed_set_serving_graphs_bit_exact(torch_cuda):
             sets[key].append(StagedClip(torch, args, clip, m))
     stream = torch.cuda.Stream()
     sp = stream.cuda_stream
-    graphs = {}
+    graphs, rgraphs = {}, {}
     warm = 3  # the first frame-loop step (bt=5: frame 2) fills the ring eagerly
     checked = 0
     with torch.cuda.stream(stream):
@@ -292,23 +295,26 @@ def test_mixed_set_serving_graphs_bit_exact(torch_cuda):
             for c in sets["single"]:
                 for it in c.items(k):
                     run_device_batch([it], sp)
-            items = [it for c in sets["graph"] for it in c.items(k)]
-            if k < warm:
-                run_device_batch(items, sp)
-            else:
-                if k < warm + S:
-                    print(f"capture phase {k % S} at frame {k}", flush=True)
-                    graphs[k % S] = Graph(items, sp)
-                graphs[k % S].launch(sp)
+            for key, gs in (("graph", graphs), ("ring", rgraphs)):
+                items = [it for c in sets[key] for it in c.items(k)]
+                if k < warm:
+                    run_device_batch(items, sp)
+                else:
+                    if k < warm + S:
+                        print(f"capture {key} phase {k % S} at frame {k}", flush=True)
+                        gs[k % S] = Graph(items, sp, ring=key == "ring")
+                    gs[k % S].launch(sp)
             stream.synchronize()
-            for a, b, g in zip(sets["batch"], sets["single"], sets["graph"]):
+            for a, b, g, r in zip(sets["batch"], sets["single"], sets["graph"], sets["ring"]):
                 for p in range(3):
                     want = a.out[p][k % S]
                     assert torch.equal(b.out[p][k % S], want), ("single", k, p)
                     assert torch.equal(g.out[p][k % S], want), ("graph", k, p)
+                    assert torch.equal(r.out[p][k % S], want), ("ring", k, p)
                     checked += 1
-    print(f"{checked} plane-frames equal; {len(graphs)} graphs, replayed over {F - 2 - warm} steps", flush=True)
-    for g in graphs.values():
+    print(f"{checked} plane-frames equal; {len(graphs)} + {len(rgraphs)} graphs, replayed over {F - 2 - warm} steps",
+          flush=True)
+    for g in list(graphs.values()) + list(rgraphs.values()):
         g.close()
     for key in sets:
         for c in sets[key]:

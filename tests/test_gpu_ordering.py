@@ -112,6 +112,34 @@ def test_eager_replay_eager(torch_cuda):
     g2.close()
 
 
+def test_ring_bound_graph_checks_the_ring(torch_cuda):
+    """A graph bound to the spectra rings (nf3d_run_device_batch_graph_ring)
+    reads the frames the eager call before its capture left there: replayed
+    after that call and followed by the next range, the clip equals one eager
+    call bit for bit.  Once the rings hold frames that are not the capture's
+    shifted alike, the replay is refused and nothing runs."""
+    from neo_fft3d_amd import Graph, Nf3dError, run_device_batch
+    torch = torch_cuda
+    clip, bufs, engines = _setup(torch)
+    want = _reference(torch, bufs, engines, 48)
+    engs = engines()
+    stream = torch.cuda.Stream()
+    s = stream.cuda_stream
+    run_device_batch(_items(engs, bufs, 0, 16), s)           # ring holds 14, 15, 16
+    g = Graph(_items(engs, bufs, 16, 16), s, ring=True)       # reads 15, 16 from the ring
+    g.launch(s)                                               # frames 16..31
+    run_device_batch(_items(engs, bufs, 32, 16), s)           # continues from the replay's ring
+    stream.synchronize()
+    _check(torch, bufs, want, "ring-bound replay")
+    before = [b[1].cpu().numpy().copy() for b in bufs]
+    with pytest.raises(Nf3dError):                            # rings hold 45, 46, 47 now
+        g.launch(s)
+    stream.synchronize()
+    for p, b in enumerate(bufs):
+        assert np.array_equal(b[1].cpu().numpy(), before[p]), p
+    g.close()
+
+
 def test_out_of_order_eager_ranges(torch_cuda):
     """Ranges run out of order eagerly: every call transforms what its ring
     does not hold."""
