@@ -443,6 +443,7 @@ def multi_clip(args):
             e.close()
     graph = serving_graphs(clips, F, models)
     graph_ring = serving_graphs(clips, F, models, ring=True)
+    graph_ring_period = serving_graphs(clips, F, models, ring=True, period=15)
     return {"workload": "16 clips: 4x 854x480 P8, 4x 1280x720 P8, 4x 1920x1080 P10, 4x 3840x2160 YUV444P16; "
                         "bt {0,1,3,5}, bw {32,48,64}, ow 10/16/22, sigma U[1.5,4]",
             "clips": [{k: c[k] for k in ("w", "h", "bits", "bt", "bw", "sigma", "learned")} for c in clips],
@@ -450,11 +451,12 @@ def multi_clip(args):
             "step_latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
             "step_latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
             "graph_replay": graph, "graph_replay_ring": graph_ring,
+            "graph_replay_ring_learned_period_15": graph_ring_period,
             "learned_psd": f"on for {sum(c['learned'] for c in clips)} clips, {len(models)} networks "
                            f"(setup incl. data and training {t_train:.1f} s)"}
 
 
-def serving_graphs(clips, F, models, ring=False):
+def serving_graphs(clips, F, models, ring=False, period=0):
     """The same 16 clips served as a stream: frame j of each plane arrives in
     staging slot j % S (S = 15, a multiple of every bt) and is filtered into
     output slot j % S, so a frame step's work repeats every S steps.  After
@@ -464,7 +466,8 @@ def serving_graphs(clips, F, models, ring=False):
     previous step left in the rings) and replayed once to advance the
     spectra ring; the timed steps are the frame's arrival (a device copy per
     plane, issued as one multi-tensor copy) plus one graph launch per phase.
-    Fresh engines, same parameters and networks."""
+    Fresh engines, same parameters and networks; `period` > 0 predicts the
+    learned PSD once per that many frames (nf3d_engine_set_learned_period)."""
     import numpy as np
     import torch
 
@@ -480,7 +483,7 @@ def serving_graphs(clips, F, models, ring=False):
                      c["h"], F, p) for p in range(3)]
         if c["learned"]:
             for e in es:
-                LRN.set_learned(e, models[(c["bw"], ow, c["bits"])], 3.0)
+                LRN.set_learned(e, models[(c["bw"], ow, c["bits"])], 3.0, period=period)
         stg = [torch.zeros((S,) + tuple(x.shape[1:]), dtype=x.dtype, device="cuda") for x in c["src"]]
         out = [torch.zeros_like(x) for x in stg]
         gcl.append(dict(src=c["src"], stg=stg, out=out, es=es))

@@ -148,11 +148,14 @@ int nf3d_run_device_batch_graph(const nf3d_run_item* items, int n, void* stream,
  * reference counterpart; for a stream of frame steps replayed in order): the
  * captured work reads the Wiener-3D spectra the rings hold instead of
  * transforming those frames again.  nf3d_graph_launch then replays it only
- * if every engine's ring holds the frames it held at capture, shifted by one
- * whole number of frames per engine (a multiple of bt: frame f sits in slot
- * f mod bt), and returns -1 without launching otherwise; the caller puts the
- * shifted frames at the captured frame pointers (staging slots reused
- * periodically, as a serving loop does). */
+ * if every engine has moved on by a whole number of frames since the
+ * capture (its latest call's last frame, a multiple of bt) and its ring holds
+ * the frames it held at capture shifted by that many (frame f sits in slot
+ * f mod bt), and a learned PSD kept from a span before the capture is the
+ * one of the span shifted alike (the shift a multiple of the period);
+ * otherwise it returns -1 without launching.  The caller puts the shifted
+ * frames at the captured frame pointers (staging slots reused periodically,
+ * as a serving loop does). */
 int nf3d_run_device_batch_graph_ring(const nf3d_run_item* items, int n, void* stream, nf3d_graph** graph,
                                      char* err, size_t errlen);
 int nf3d_graph_launch(nf3d_graph* graph, void* stream);
@@ -247,11 +250,14 @@ int nf3d_engine_set_learned(nf3d_engine* e, nf3d_mlp* m, float strength, char* e
 /* Prediction period of the learned PSD in the frame-loop path (no reference
  * counterpart; the reference's own pattern is estimated once, from frame
  * pframe, for the whole clip): 0 (default) = one prediction per launch, as
- * above; n > 0 = one per span of n frames, the spans starting at the
- * multiples of n and at the first frame-loop frame of each call, every span
- * filtered with the PSD of its first frame.  Launches are split at the span
- * starts, so the result no longer depends on frames per launch.  Errors:
- * n < 0. */
+ * above; n > 0 = one per span of frames [k n, (k + 1) n), made from the
+ * span's first frame-loop frame that a call reaches and kept for the calls
+ * that continue the span (in-order calls of any length give the PSD of
+ * frame max(k n, first frame-loop frame) to every frame-loop frame of span
+ * k).  Launches are split at the span starts, so the result no longer
+ * depends on frames per launch.  A per-frame call, a new model or period,
+ * or a graph capture not bound to the rings (which predicts at its first
+ * launch) starts over.  Errors: n < 0. */
 int nf3d_engine_set_learned_period(nf3d_engine* e, int frames, char* err, size_t errlen);
 
 /* Spatially varying noise pattern (no reference counterpart; it generalises

@@ -206,9 +206,15 @@ struct nf3d_engine {
   nf3d_mlp* learned = nullptr;
   float learned_strength = 1.0f;
   // frame-loop predictions: 0 = one per launch (from its first frame); n > 0
-  // = one per span of n frames (spans start at multiples of n and at the
-  // range's first frame-loop frame; launches never cross a span's start)
+  // = one per span of n frames [k n, (k + 1) n) (launches never cross a
+  // span's start), from the span's first frame-loop frame of the call that
+  // first reaches it; lspan = the span whose PSD d_lpat holds (-1: none), so
+  // a call continuing a span reuses it
   int learned_period = 0;
+  long long lspan = -1;
+  // last frame of the latest device-range call (-1 after a per-frame call):
+  // the frame shift a ring-bound graph replay is checked against
+  int last_end = -1;
   float2* d_lspec = nullptr;
   void* d_lx = nullptr;
   float* d_lpat = nullptr;
@@ -874,6 +880,7 @@ struct nf3d_engine {
 
   // ... from spectra already computed (a frame in the Wiener-3D ring)
   void learned_psd_spec(const float2* spec, float mult, hipStream_t s) {
+    lspan = -1;
     if (nf3d_mlp_features(learned, spec, nb, d_lx, s)) fail("learned PSD: feature kernel");
     char msg[256] = {0};
     if (nf3d_mlp_predict(learned, d_lx, nb, d_lpat, 1, mult * learned_strength, d_lwork, s, msg, sizeof msg))
@@ -1066,6 +1073,7 @@ struct nf3d_engine {
       spans.push_back({P.a, P.b});
     }
     int j = 0;
+    long long cur = lspan;
     for (const auto& sp : spans) {
       const int total = sp.second - sp.first + 1;
       const int nl = (total + run_frames - 1) / run_frames;
@@ -1092,7 +1100,9 @@ struct nf3d_engine {
         R.fpw = (P.mode == nf3d::MODE_W3D || P.mode == nf3d::MODE_KALMAN) ? nf : std::min(run_fpw, nf);
         R.ntiles = noy * ntx;
         P.launches.push_back(R);
-        P.predict.push_back(learned_period <= 0 || jj == 0);
+        const long long idx = learned_period > 0 ? sp.first / learned_period : -1;
+        P.predict.push_back(learned_period <= 0 || (jj == 0 && idx != cur));
+        if (learned_period > 0) cur = idx;
         m0 += nf;
       }
     }
@@ -1261,6 +1271,7 @@ int nf3d_engine_get_frame(nf3d_engine* e, int n, const nf3d_frame_ref* frames, i
                           int dst_stride, char* err, size_t errlen) {
   std::lock_guard<std::mutex> lock(e->mu);
   try {
+    e->last_end = -1;
     e->get_frame_host(n, frames, nframes, dst, dst_stride);
   } catch (const Failure& f) {
     copy_err(err, errlen, f.msg);
@@ -1275,6 +1286,7 @@ int nf3d_engine_get_frame_device(nf3d_engine* e, int n, const nf3d_frame_ref* fr
   try {
     nf3d_engine::Call c{frames, nframes, true, stream ? static_cast<hipStream_t>(stream) : e->stream};
     e->order_before(c.s);
+    e->last_end = -1;
     e->run(n, c, static_cast<uint8_t*>(dst), dst_stride);
     e->order_after(c.s);
   } catch (const Failure& f) {
@@ -1337,8 +1349,11 @@ void learned_predictions(const nf3d_run_item* items, const std::vector<nf3d_engi
       e2->launch(nf3d::MODE_FWD, 0, F, s);
       return e2->d_lspec;
     };
-    if (group.size() == 1) {
+    // (the clip-end launches predict into the batch buffer even alone, so
+    // that d_lpat keeps a span's PSD for the calls that continue it)
+    if (group.size() == 1 && l != SIZE_MAX) {
       e->learned_psd_spec(spectra(c), mult, s);
+      if (l != SIZE_MAX && e->learned_period > 0) e->lspan = B.job[c].m0 / e->learned_period;
       continue;
     }
     long long rows = 0;
@@ -1383,10 +1398,12 @@ void learned_predictions(const nf3d_run_item* items, const std::vector<nf3d_engi
     row = 0;
     for (int c2 : group) {
       nf3d_engine* e2 = items[members[c0 + c2]].engine;
-      if (l != SIZE_MAX && e2->learned_period > 0)
+      if (l != SIZE_MAX && e2->learned_period > 0) {
         check(hipMemcpyAsync(e2->d_lpat, e->d_bp + row * n, sizeof(float) * (size_t)(e2->nb * n),
                              hipMemcpyDeviceToDevice, s),
               "learned PSD: copy");
+        e2->lspan = B.job[c2].m0 / e2->learned_period;
+      }
       row += e2->nb;
     }
   }
@@ -1491,6 +1508,7 @@ void run_batch(const nf3d_run_item* items, int n, hipStream_t s, bool timed, boo
     }
   }
   for (int i = 0; i < n; i++) items[i].engine->run_epilogue(plans[i], s);
+  for (int i = 0; i < n; i++) items[i].engine->last_end = items[i].first + items[i].count - 1;
   if (!capture)
     for (int i = 0; i < n; i++) items[i].engine->order_after(s);
 }
@@ -1632,6 +1650,10 @@ struct nf3d_graph {
   bool ring_bound = false;
   std::vector<std::vector<int>> ring_before;
   std::vector<std::vector<char>> by_run_before;
+  // each engine's last frame (last_end) and learned-PSD span (lspan) at
+  // capture and after one replay
+  std::vector<int> end_before, end_after;
+  std::vector<long long> lspan_before, lspan_after;
 };
 
 int nf3d_run_host_batch(const nf3d_run_item* items, int n, char* err, size_t errlen) {
@@ -1910,11 +1932,18 @@ int capture_batch(const nf3d_run_item* items, int n, void* stream, nf3d_graph** 
     // the captured work reads the frames they hold, and a replay checks them)
     std::vector<std::vector<int>> before(n);
     std::vector<std::vector<char>> by_before(n);
+    std::vector<int> end_before(n);
+    std::vector<long long> lspan_before(n);
     for (int i = 0; i < n; i++) {
       nf3d_engine* e = items[i].engine;
       before[i] = e->ring_frame;
       by_before[i] = e->ring_by_run;
-      if (!keep_ring) std::fill(e->ring_frame.begin(), e->ring_frame.end(), -1);
+      end_before[i] = e->last_end;
+      lspan_before[i] = e->lspan;
+      if (!keep_ring) {
+        std::fill(e->ring_frame.begin(), e->ring_frame.end(), -1);
+        e->lspan = -1;  // and predicts the learned PSD itself
+      }
     }
     check(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
     capturing = true;
@@ -1928,6 +1957,12 @@ int capture_batch(const nf3d_run_item* items, int n, void* stream, nf3d_graph** 
       g->engines.push_back(e);
       g->ring_after.push_back(e->ring_frame);
       g->by_run_after.push_back(e->ring_by_run);
+      g->end_before.push_back(end_before[i]);
+      g->end_after.push_back(e->last_end);
+      g->lspan_before.push_back(lspan_before[i]);
+      g->lspan_after.push_back(e->lspan);
+      e->last_end = end_before[i];
+      e->lspan = lspan_before[i];
       if (keep_ring) {
         e->ring_frame = before[i];
         e->ring_by_run = by_before[i];
@@ -1962,26 +1997,32 @@ int capture_batch(const nf3d_run_item* items, int n, void* stream, nf3d_graph** 
   return 0;
 }
 
-// The frames by which each engine's ring is ahead of what a ring-bound graph
-// was captured against (every slot the capture saw shifted alike, the
-// slots' kernel-written flags equal), or false.
-bool ring_shift(const nf3d_graph* g, std::vector<int>& shift) {
+// The frames by which each engine has moved on since a ring-bound graph's
+// capture (its last frame then and now), if what the captured work reads
+// is there: every ring slot the capture saw holds its frame shifted by that
+// many frames, with the same kernel-written flag (a multiple of bt: the
+// frame loop's slot of frame f is f mod bt), and a learned PSD the capture
+// reused is the one of the span shifted alike.  False otherwise.
+bool replay_shift(const nf3d_graph* g, std::vector<int>& shift) {
   shift.assign(g->engines.size(), 0);
   for (size_t i = 0; i < g->engines.size(); i++) {
     const nf3d_engine* e = g->engines[i];
     const std::vector<int>& was = g->ring_before[i];
-    if (e->ring_frame.size() != was.size()) return false;
-    bool have = false;
-    for (size_t k = 0; k < was.size(); k++) {
-      if (was[k] < 0) continue;
-      const int d = e->ring_frame[k] - was[k];
-      if (e->ring_frame[k] < 0 || e->ring_by_run[k] != g->by_run_before[i][k] || (have && d != shift[i]))
+    bool relies = g->lspan_before[i] >= 0;
+    for (int f : was) relies = relies || f >= 0;
+    const bool known = e->last_end >= 0 && g->end_before[i] >= 0;
+    const int d = known ? e->last_end - g->end_before[i] : 0;
+    shift[i] = d;
+    if (!relies) continue;
+    if (!known || e->ring_frame.size() != was.size()) return false;
+    if (e->ep.bt >= 2 && d % e->ep.bt != 0) return false;
+    for (size_t k = 0; k < was.size(); k++)
+      if (was[k] >= 0 && (e->ring_frame[k] != was[k] + d || e->ring_by_run[k] != g->by_run_before[i][k]))
         return false;
-      shift[i] = d;
-      have = true;
+    if (g->lspan_before[i] >= 0) {
+      const int per = e->learned_period;
+      if (per <= 0 || d % per != 0 || e->lspan != g->lspan_before[i] + d / per) return false;
     }
-    // the frame loop's slot of frame f is f mod bt
-    if (have && e->ep.bt >= 2 && shift[i] % e->ep.bt != 0) return false;
   }
   return true;
 }
@@ -2005,7 +2046,7 @@ int nf3d_graph_launch(nf3d_graph* g, void* stream) {
   for (nf3d_engine* e : es) locks.emplace_back(e->mu);
   hipStream_t s = static_cast<hipStream_t>(stream);
   std::vector<int> shift(g->engines.size(), 0);
-  if (g->ring_bound && !ring_shift(g, shift)) return -1;
+  if (g->ring_bound && !replay_shift(g, shift)) return -1;
   try {
     for (nf3d_engine* e : g->engines) e->order_before(s);
     check(hipGraphLaunch(g->exec, s), "hipGraphLaunch");
@@ -2021,6 +2062,9 @@ int nf3d_graph_launch(nf3d_graph* g, void* stream) {
     for (int& f : e->ring_frame)
       if (f >= 0) f += shift[i];
     e->ring_by_run = g->by_run_after[i];
+    e->last_end = g->end_after[i] + shift[i];
+    e->lspan = g->lspan_after[i];
+    if (e->lspan >= 0 && e->learned_period > 0) e->lspan += shift[i] / e->learned_period;
   }
   return 0;
 }
@@ -2101,6 +2145,7 @@ int nf3d_engine_set_learned(nf3d_engine* e, nf3d_mlp* m, float strength, char* e
     e->d_lwork = dev_alloc<uint8_t>((size_t)nf3d_mlp_work_bytes(m, e->nb));
     e->learned = m;
     e->learned_strength = strength;
+    e->lspan = -1;
     // the learned PSD is predicted once per frame-loop launch, so frames per
     // launch is part of its result: a tuning-table entry (chosen for speed)
     // does not apply to it; only nf3d_engine_set_tuning changes it
@@ -2120,6 +2165,7 @@ int nf3d_engine_set_learned_period(nf3d_engine* e, int frames, char* err, size_t
   try {
     if (frames < 0) fail("learned PSD: the prediction period must be 0 (one per launch) or a frame count");
     e->learned_period = frames;
+    e->lspan = -1;
   } catch (const Failure& f) {
     copy_err(err, errlen, f.msg);
     return -1;

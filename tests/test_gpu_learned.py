@@ -329,17 +329,20 @@ def psd_source_frame(n, bt, F, per_launch, period=0):
     raise AssertionError("frame outside the launches")
 
 
-@pytest.mark.parametrize("family,bits,is_float,bt,per_launch,period", [
-    ("rgb", 32, True, 3, 1, 0), ("gray", 8, False, 1, 1, 0), ("gray", 8, False, 0, 1, 0), ("rgb", 32, True, 5, 1, 0),
-    ("rgb", 32, True, 3, 16, 0), ("gray", 8, False, 1, 4, 0), ("gray", 8, False, 0, 3, 0),
-    ("rgb", 32, True, 3, 2, 4), ("gray", 8, False, 1, 3, 4), ("gray", 8, False, 0, 16, 3)])
-def test_learned_path_matches_oracle(torch, family, bits, is_float, bt, per_launch, period):
+@pytest.mark.parametrize("family,bits,is_float,bt,per_launch,period,per_call", [
+    ("rgb", 32, True, 3, 1, 0, 0), ("gray", 8, False, 1, 1, 0, 0), ("gray", 8, False, 0, 1, 0, 0),
+    ("rgb", 32, True, 5, 1, 0, 0), ("rgb", 32, True, 3, 16, 0, 0), ("gray", 8, False, 1, 4, 0, 0),
+    ("gray", 8, False, 0, 3, 0, 0), ("rgb", 32, True, 3, 2, 4, 0), ("gray", 8, False, 1, 3, 4, 0),
+    ("gray", 8, False, 0, 16, 3, 0), ("rgb", 32, True, 3, 16, 4, 2), ("gray", 8, False, 0, 16, 3, 1),
+    ("gray", 8, False, 1, 2, 4, 3)])
+def test_learned_path_matches_oracle(torch, family, bits, is_float, bt, per_launch, period, per_call):
     """The engine's learned-PSD filtering against the FP64 oracle given the
     same per-block PSDs (the model's exp(log-PSD) of the frame the engine
     predicts from): per-block pattern indexing in both kernels, the exp
     epilogue, the btcur scaling, the one-PSD-per-launch rule and, with a
     prediction period, the one-PSD-per-span rule (launches of a span after
-    its first keep its PSD)."""
+    its first keep its PSD, also across calls: per_call > 0 runs the clip as
+    in-order calls of that many frames)."""
     import ctypes
     from harness import Clip, compare, oracle_runner
     from neo_fft3d_amd import Engine, default_params, make_format
@@ -402,8 +405,10 @@ def test_learned_path_matches_oracle(torch, family, bits, is_float, bt, per_laun
         dst = torch.zeros_like(srcs[p])
         torch.cuda.synchronize()  # the engine's stream does not wait for torch's
         stride = srcs[p][0].stride(0) * srcs[p].element_size()
-        engines[p].run_device(0, F, [srcs[p][n].data_ptr() for n in range(F)], stride,
-                              [dst[n].data_ptr() for n in range(F)], stride)
+        step = per_call or F
+        for a in range(0, F, step):
+            engines[p].run_device(a, min(step, F - a), [srcs[p][n].data_ptr() for n in range(F)], stride,
+                                  [dst[n].data_ptr() for n in range(F)], stride)
         engines[p].sync()
         gpu.append(dst.cpu().numpy())
     # the oracle with the same PSDs, frame by frame
@@ -417,7 +422,7 @@ def test_learned_path_matches_oracle(torch, family, bits, is_float, bt, per_laun
         want = h.get(n)
         got = [gpu[p][n] for p in range(nplanes)]
         for p, (mx, fe, rms) in enumerate(compare(got, want, is_float)):
-            print(f"learned {family}{bits} bt={bt} per_launch={per_launch} period={period} n={n} "
+            print(f"learned {family}{bits} bt={bt} per_launch={per_launch} period={period} per_call={per_call} n={n} "
                   f"(PSD of frame {src_n}) p={p}: "
                   f"max {mx:.3g} exact {fe:.5f} rms {rms:.3g} "
                   f"(log-PSD spread {spread:.2f})", flush=True)
