@@ -718,6 +718,9 @@ def learned_on_metric_config(args):
             e.sync()
         t = time.perf_counter()
         for _ in range(reps):
+            if learned and period:
+                for e in es:  # each pass predicts afresh (no span kept from the pass before)
+                    L.set_learned_period(e, period)
             run_device_batch(items)
         for e in es:
             e.sync()

@@ -214,7 +214,15 @@ def set_learned(engine: Engine, mlp: Mlp | None, strength: float = 1.0, period: 
     if _bind().nf3d_engine_set_learned(engine.h, mlp.h if mlp is not None else None, ctypes.c_float(strength),
                                        err, 512) != 0:
         raise Nf3dError(err.value.decode())
-    if period is not None and _bind().nf3d_engine_set_learned_period(engine.h, int(period), err, 512) != 0:
+    if period is not None:
+        set_learned_period(engine, period)
+
+
+def set_learned_period(engine: Engine, period: int):
+    """nf3d_engine_set_learned_period: one frame-loop prediction per launch
+    (0) or per span of `period` frames; also forgets the kept span."""
+    err = ctypes.create_string_buffer(512)
+    if _bind().nf3d_engine_set_learned_period(engine.h, int(period), err, 512) != 0:
         raise Nf3dError(err.value.decode())
 
 
