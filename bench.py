@@ -360,7 +360,7 @@ def precision_filter(args):
             fps_m, out_m = run(bt, bw, 1)
             d = [np.abs(a.astype(np.int32) - b.astype(np.int32)) for a, b in zip(out_fft, out_m)]
             row.update({"fps_mfma_dft": round(fps_m, 1), "mfma_over_fft": round(fps_m / fps, 3),
-                        "mfma_kernel": "frame loop" if bw == 32 else "per-frame",
+                        "mfma_kernel": "frame loop" if bw in (32, 48) else "per-frame",
                         "forms_max_diff_lsb": int(max(x.max() for x in d)),
                         "forms_equal_frac": round(float(np.mean([np.mean(x == 0) for x in d])), 6)})
             res.append(row)

@@ -1051,9 +1051,9 @@ struct nf3d_engine {
                       (long long)(tc + Hc) * dst_pitch < (1LL << 31) && 4LL * Hc * Wc < (1LL << 31) &&
                       dst_pitch % pix == 0;
     const bool pshow = ep.pfactor != 0 && ep.pshow;
-    // the matrix-core form of 48x48 and 64x64 blocks runs in the per-frame
-    // kernel only (in the frame loop its operands would not fit the registers)
-    const bool mf_tile = dft_mode && ep.bw != 32;
+    // the matrix-core form of 64x64 blocks runs in the per-frame kernel only
+    // (in the frame loop its operands would not fit the registers)
+    const bool mf_tile = dft_mode && ep.bw == 64;
     P.run = P.a <= P.b && fits && !pshow && !mf_tile && !(P.mode == nf3d::MODE_SHARPEN && P.shp == nf3d::SHP_NONE);
     if (!P.run) return P;
     // launches of equal length (within one frame), two halves of seam

@@ -521,7 +521,7 @@ __device__ __forceinline__ void split_combine_inv(float2 (&c)[BH / 2], int h, fl
 // the frame loop.
 //
 // MF: the 2D transforms as DFT-matrix products on the matrix cores (32x32
-// blocks, nf3d_mfma.h), a wavefront per block in turn, in place of the row
+// and 48x48 blocks, nf3d_mfma.h), a wavefront per block in turn, in place of the row
 // and column FFTs; the spectral pass is unchanged.
 //
 // NT = 256 (FFT form): the tile of the 128-lane geometry (BPW blocks) with
@@ -535,10 +535,12 @@ __device__ __forceinline__ void split_combine_inv(float2 (&c)[BH / 2], int h, fl
 // workgroups per CU at 48x48): three, or two for the variants whose spectral
 // pass holds more per element (four or five frames, the dehalo factor, the
 // Kalman state with a sharpen pass, 64-row columns), which would otherwise
-// spill.
-template <int BH, int NT, int MODE, int BT, int SHP>
+// spill; one for the matrix-core form at 48x48 (256 VGPRs and the AGPRs,
+// no spills).
+template <int BH, int NT, int MODE, int BT, int SHP, bool MF>
 constexpr int run_blocks_per_cu() {
-  if constexpr (NT != 256) return 2;
+  if constexpr (MF && BH >= 48) return 1;
+  else if constexpr (NT != 256) return 2;
   else if constexpr (BH >= 64 || (MODE == MODE_W3D && (BT >= 4 || SHP == SHP_GENERAL)) ||
                      (MODE == MODE_KALMAN && SHP != SHP_NONE))
     return 2;
@@ -546,7 +548,7 @@ constexpr int run_blocks_per_cu() {
 }
 
 template <int BW, int BH, int NT, int MODE, int BT, int SHP, bool MF = false>
-__global__ void __launch_bounds__(NT, (run_blocks_per_cu<BH, NT, MODE, BT, SHP>())) k_run(RunBatch B_arg) {
+__global__ void __launch_bounds__(NT, (run_blocks_per_cu<BH, NT, MODE, BT, SHP, MF>())) k_run(RunBatch B_arg) {
   const RunBatch& B0 = kernarg_batch(B_arg);
   const int jb = batch_job(B0, blockIdx.x);
   const RunLaunch& L0 = B0.job[jb];
@@ -695,7 +697,7 @@ __global__ void __launch_bounds__(NT, (run_blocks_per_cu<BH, NT, MODE, BT, SHP>(
       // gather stage of blocks b and b+1 (static_assert below), so a round
       // writes over stages its own round or earlier rounds have read, and one
       // barrier per round (operands loaded by both wavefronts) is enough.
-      static_assert(BW == BH && (BW == 32 || BW == 64), "matrix-core DFT: 32x32 or 64x64 blocks");
+      static_assert(BW == BH && (BW == 32 || BW == 48 || BW == 64), "matrix-core DFT: 32x32, 48x48 or 64x64 blocks");
       static_assert(LSPEC >= LA && 2 * LA - LSPEC >= (BPW - 1) * (LSPEC - LA), "spectra overlap stages b, b+1 only");
       constexpr int NW = NT / 64, MB = (BPW + NW - 1) / NW;
       const int wave = wave_of(t), lane = t & 63;
@@ -1192,7 +1194,7 @@ hipError_t launch_run_modes(int mode, int bt, int shp, const RunBatch& L, hipStr
   return hipGetLastError();
 }
 
-// mf: the matrix-core DFT form (32x32 blocks only).
+// mf: the matrix-core DFT form (32x32 and 48x48 blocks).
 template <int BW, int BH>
 hipError_t launch_run_shape(int mode, int bt, int shp, int mf, const RunBatch& L, hipStream_t s) {
   if (L.njobs < 1 || L.njobs > RUN_MAX_JOBS) return hipErrorInvalidValue;
@@ -1207,7 +1209,7 @@ hipError_t launch_run_shape(int mode, int bt, int shp, int mf, const RunBatch& L
   if (mf) {
     // 64x64 blocks in the matrix-core form go through the per-frame kernel
     // (in the frame loop their operands would exceed the register file)
-    if constexpr (BW == 32 && BH == 32) return launch_run_modes<BW, BH, true>(mode, bt, shp, L, s);
+    if constexpr (BW == BH && (BW == 32 || BW == 48)) return launch_run_modes<BW, BH, true>(mode, bt, shp, L, s);
     else return hipErrorInvalidValue;
   }
   return launch_run_modes<BW, BH, false>(mode, bt, shp, L, s);
