@@ -538,7 +538,7 @@ __device__ __forceinline__ void split_combine_inv(float2 (&c)[BH / 2], int h, fl
 // spill; one for the matrix-core form at 48x48 (256 VGPRs and the AGPRs,
 // no spills).
 template <int BH, int NT, int MODE, int BT, int SHP, bool MF>
-constexpr int run_blocks_per_cu() {
+constexpr int run_waves_per_simd() {
   if constexpr (MF && BH >= 48) return 1;
   else if constexpr (NT != 256) return 2;
   else if constexpr (BH >= 64 || (MODE == MODE_W3D && (BT >= 4 || SHP == SHP_GENERAL)) ||
@@ -548,7 +548,7 @@ constexpr int run_blocks_per_cu() {
 }
 
 template <int BW, int BH, int NT, int MODE, int BT, int SHP, bool MF = false>
-__global__ void __launch_bounds__(NT, (run_blocks_per_cu<BH, NT, MODE, BT, SHP, MF>())) k_run(RunBatch B_arg) {
+__global__ void __launch_bounds__(NT, (run_waves_per_simd<BH, NT, MODE, BT, SHP, MF>())) k_run(RunBatch B_arg) {
   const RunBatch& B0 = kernarg_batch(B_arg);
   const int jb = batch_job(B0, blockIdx.x);
   const RunLaunch& L0 = B0.job[jb];
