@@ -2,8 +2,8 @@
 
 The frame-loop kernel (k_run) filters a run of frames per launch.  Two launch
 shapes are free, because no choice of them changes a single output bit
-(engines with a learned PSD excepted: their PSD is predicted once per launch,
-so table entries are not applied to them):
+(engines with a learned PSD excepted: their PSD is predicted once per launch
+unless a prediction period is set, so table entries are not applied to them):
 
   frames per launch       1..16; more frames amortise the launch and the
                           per-launch seam pass, fewer start the next launch's
