@@ -614,9 +614,10 @@ def learned_form(args):
     dst = [torch.zeros_like(s_) for s_ in src]
     stride = 1920 * 4
 
-    def timed(es, batch):
+    def timed(es, batch, period=0):
         # both forms through nf3d_run_device_batch; the learned form predicts
-        # each plane's per-block PSD once per frame-loop launch (16 frames)
+        # each plane's per-block PSD once per frame-loop launch (16 frames),
+        # or once per `period` frames (each timed pass predicting afresh)
         items = [(e, 0, F, [src[p][k].data_ptr() for k in range(F)], stride, [dst[p][k].data_ptr() for k in range(F)],
                   stride) for p, e in enumerate(es)]
         torch.cuda.synchronize()
@@ -629,6 +630,9 @@ def learned_form(args):
             e.sync()
         t = time.perf_counter()
         for _ in range(2):
+            if period:
+                for e in es:
+                    L.set_learned_period(e, period)
             if batch:
                 run_device_batch(items)
             else:
@@ -644,6 +648,7 @@ def learned_form(args):
         e.close()
     es = engines(1920, 1080, F, 2.0, mlp, tuned["learned_noise_labels"])
     fps_learned = timed(es, True)
+    fps_learned_period = {str(per): round(timed(es, True, per), 1) for per in (64, F)}
     for e in es:
         e.close()
     # MLP inference rate alone (one 1080p plane's blocks)
@@ -673,6 +678,7 @@ def learned_form(args):
         "quality": {"clips": 4, "size": f"{QW}x{QH}x{QF} frames", "tuned_on": "one held-out clip", **quality},
         "psnr_gain_db": quality["learned_noise_labels"]["gain_over_plain_db"],
         "fps_1080p_plain": round(fps_plain, 1), "fps_1080p_learned": round(fps_learned, 1),
+        "fps_1080p_learned_by_prediction_period": fps_learned_period,
         "timing_frames": F,
         "mlp_forward": {"blocks": nb, "ms": round(ms, 4), "tflops_bf16": round(flops / (ms * 1e-3) / 1e12, 1),
                         "frac_of_2500": round(flops / (ms * 1e-3) / 1e12 / 2500.0, 4)},
