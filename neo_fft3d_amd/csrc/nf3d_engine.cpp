@@ -1947,7 +1947,19 @@ int capture_batch(const nf3d_run_item* items, int n, void* stream, nf3d_graph** 
     }
     check(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
     capturing = true;
-    run_batch(items, n, s, false, true);
+    try {
+      run_batch(items, n, s, false, true);
+    } catch (const Failure&) {
+      // a capture cut short may have recorded frames in the rings or a kept
+      // learned PSD that nothing computed: forget them (the locks still held)
+      for (int i = 0; i < n; i++) {
+        nf3d_engine* e = items[i].engine;
+        std::fill(e->ring_frame.begin(), e->ring_frame.end(), -1);
+        e->lspan = -1;
+        e->last_end = -1;
+      }
+      throw;
+    }
     auto* g = new nf3d_graph;
     capturing = false;
     g->ring_bound = keep_ring;
