@@ -1011,8 +1011,9 @@ struct nf3d_engine {
     std::vector<char> predict;
     // Wiener 3D: the clip-end frames of the range (btcur 1, the 2D gains)
     // as one-frame launches of the frame-loop kernel in its 2D mode, instead
-    // of the per-frame path
-    std::vector<RunLaunch> edges;
+    // of the per-frame path: those before the frame-loop frames (run after
+    // the prologue, while the ring still holds them) and those after
+    std::vector<RunLaunch> edges_start, edges;
   };
 
   RangePlan plan_range(int first, int count, const void* const* src, long long src_pitch, void* const* dst,
@@ -1122,7 +1123,7 @@ struct nf3d_engine {
         R.nframes = 1;
         R.fpw = 1;
         R.ntiles = noy * ntx;
-        P.edges.push_back(R);
+        (n < P.a ? P.edges_start : P.edges).push_back(R);
       }
     }
     return P;
@@ -1133,7 +1134,7 @@ struct nf3d_engine {
   void run_prologue(const RangePlan& P, hipStream_t s) {
     check(hipSetDevice(device), "hipSetDevice");
     const int stop = P.run ? P.a : P.first + P.count;
-    if (P.edges.empty())
+    if (P.edges.empty() && P.edges_start.empty())
       for (int n = P.first; n < stop; n++) run_one(n, P.src, P.src_pitch, P.dst, P.dst_pitch, s);
     if (!P.run) return;
     if (ep.pfactor != 0 && !pattern_set && !learned) {
@@ -1190,7 +1191,7 @@ struct nf3d_engine {
         ring_tick[k % ep.bt] = ++tick;
       }
     }
-    if (P.edges.empty())
+    if (P.edges.empty() && P.edges_start.empty())
       for (int n = P.b + 1; n < P.first + P.count; n++) run_one(n, P.src, P.src_pitch, P.dst, P.dst_pitch, s);
   }
 
@@ -1342,6 +1343,10 @@ void learned_predictions(const nf3d_run_item* items, const std::vector<nf3d_engi
       nf3d_engine* e2 = items[k2].engine;
       const RunLaunch& R = B.job[c2];
       if (mode == nf3d::MODE_W3D && e2->ep.bt >= 3) return e2->slot_ptr(R.m0 % e2->ep.bt);
+      // a Wiener-3D range's clip-end frame: in the ring when its launch runs
+      // (the start ones right after the prologue, the end ones after the
+      // frame-loop launches)
+      if (l == SIZE_MAX && plans[k2].mode == nf3d::MODE_W3D) return e2->slot_ptr(R.m0 % e2->ep.bt);
       PlaneJob F = e2->base_job();
       F.src = static_cast<const uint8_t*>(plans[k2].src[R.m0]);
       F.src_pitch = plans[k2].src_pitch;
@@ -1417,7 +1422,56 @@ void run_batch(const nf3d_run_item* items, int n, hipStream_t s, bool timed, boo
     const nf3d_run_item& it = items[i];
     plans[i] = it.engine->plan_range(it.first, it.count, it.src, it.src_stride, it.dst, it.dst_stride);
   }
+  // the clip-end frames of Wiener-3D ranges: one-frame launches of the
+  // frame-loop kernel in its 2D mode, engines of the same shape, sharpen
+  // case and transform form together (those before the frame-loop frames
+  // right after the prologues, the others after the frame-loop launches)
+  auto run_edges = [&](std::vector<RunLaunch> nf3d_engine::RangePlan::*which) {
+    std::vector<bool> edone(n, false);
+    for (int i = 0; i < n; i++) {
+      if (edone[i] || (plans[i].*which).empty()) continue;
+      nf3d_engine* e0 = items[i].engine;
+      std::vector<int> group;
+      for (int k = i; k < n; k++) {
+        nf3d_engine* e = items[k].engine;
+        if (!edone[k] && !(plans[k].*which).empty() && e->ep.bw == e0->ep.bw && e->ep.bh == e0->ep.bh &&
+            plans[k].shp == plans[i].shp && e->dft_mode == e0->dft_mode) {
+          group.push_back(k);
+          edone[k] = true;
+        }
+      }
+      size_t nq = 0;
+      for (int k : group) nq = std::max(nq, (plans[k].*which).size());
+      for (size_t q = 0; q < nq; q++) {
+        std::vector<int> members;
+        for (int k : group)
+          if (q < (plans[k].*which).size()) members.push_back(k);
+        for (size_t c0 = 0; c0 < members.size(); c0 += nf3d::RUN_MAX_JOBS) {
+          nf3d::RunBatch B{};
+          std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
+          for (size_t c = c0; c < members.size() && c < c0 + nf3d::RUN_MAX_JOBS; c++) {
+            const int k = members[c];
+            nf3d_engine* e = items[k].engine;
+            RunLaunch& R = B.job[B.njobs++];
+            R = (plans[k].*which)[q];
+            R.wg_begin = B.total_wg;
+            B.total_wg += R.ntiles;
+            B.max_hc = std::max(B.max_hc, e->Hc);
+            B.max_frames = 1;
+          }
+          learned_predictions(items, plans, members, c0, B, nf3d::MODE_W2D, SIZE_MAX, s);
+          for (int c = 0; c < B.njobs; c++) evs.push_back(items[members[c0 + c]].engine->timer(timed, 1, s));
+          check(nf3d::launch_run(nf3d::MODE_W2D, 0, plans[i].shp, e0->dft_mode, e0->ep.bw, e0->ep.bh, B, s),
+                "frame-loop kernel (clip ends)");
+          check(nf3d::launch_seam_run(e0->ep.bw, e0->ep.bh, B, s), "seam kernel");
+          for (auto& ev : evs)
+            if (ev.second) check(hipEventRecord(ev.second, s), "hipEventRecord");
+        }
+      }
+    }
+  };
   for (int i = 0; i < n; i++) items[i].engine->run_prologue(plans[i], s);
+  run_edges(&nf3d_engine::RangePlan::edges_start);
   std::vector<bool> done(n, false);
   for (int i = 0; i < n; i++) {
     if (done[i] || !plans[i].run) continue;
@@ -1462,51 +1516,7 @@ void run_batch(const nf3d_run_item* items, int n, hipStream_t s, bool timed, boo
       }
     }
   }
-  // the clip-end frames of Wiener-3D ranges: one-frame launches of the
-  // frame-loop kernel in its 2D mode, engines of the same shape, sharpen
-  // case and transform form together
-  std::vector<bool> edone(n, false);
-  for (int i = 0; i < n; i++) {
-    if (edone[i] || plans[i].edges.empty()) continue;
-    nf3d_engine* e0 = items[i].engine;
-    std::vector<int> group;
-    for (int k = i; k < n; k++) {
-      nf3d_engine* e = items[k].engine;
-      if (!edone[k] && !plans[k].edges.empty() && e->ep.bw == e0->ep.bw && e->ep.bh == e0->ep.bh &&
-          plans[k].shp == plans[i].shp && e->dft_mode == e0->dft_mode) {
-        group.push_back(k);
-        edone[k] = true;
-      }
-    }
-    size_t nq = 0;
-    for (int k : group) nq = std::max(nq, plans[k].edges.size());
-    for (size_t q = 0; q < nq; q++) {
-      std::vector<int> members;
-      for (int k : group)
-        if (q < plans[k].edges.size()) members.push_back(k);
-      for (size_t c0 = 0; c0 < members.size(); c0 += nf3d::RUN_MAX_JOBS) {
-        nf3d::RunBatch B{};
-        std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
-        for (size_t c = c0; c < members.size() && c < c0 + nf3d::RUN_MAX_JOBS; c++) {
-          const int k = members[c];
-          nf3d_engine* e = items[k].engine;
-          RunLaunch& R = B.job[B.njobs++];
-          R = plans[k].edges[q];
-          R.wg_begin = B.total_wg;
-          B.total_wg += R.ntiles;
-          B.max_hc = std::max(B.max_hc, e->Hc);
-          B.max_frames = 1;
-        }
-        learned_predictions(items, plans, members, c0, B, nf3d::MODE_W2D, SIZE_MAX, s);
-        for (int c = 0; c < B.njobs; c++) evs.push_back(items[members[c0 + c]].engine->timer(timed, 1, s));
-        check(nf3d::launch_run(nf3d::MODE_W2D, 0, plans[i].shp, e0->dft_mode, e0->ep.bw, e0->ep.bh, B, s),
-              "frame-loop kernel (clip ends)");
-        check(nf3d::launch_seam_run(e0->ep.bw, e0->ep.bh, B, s), "seam kernel");
-        for (auto& ev : evs)
-          if (ev.second) check(hipEventRecord(ev.second, s), "hipEventRecord");
-      }
-    }
-  }
+  run_edges(&nf3d_engine::RangePlan::edges);
   for (int i = 0; i < n; i++) items[i].engine->run_epilogue(plans[i], s);
   for (int i = 0; i < n; i++) items[i].engine->last_end = items[i].first + items[i].count - 1;
   if (!capture)
