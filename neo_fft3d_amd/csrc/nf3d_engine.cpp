@@ -1414,7 +1414,10 @@ void learned_predictions(const nf3d_run_item* items, const std::vector<nf3d_engi
   }
 }
 
-void run_batch(const nf3d_run_item* items, int n, hipStream_t s, bool timed, bool capture = false) {
+// keep_events: under graph capture, the fork and join events are handed
+// there, to be destroyed after the capture ends, instead of here.
+void run_batch(const nf3d_run_item* items, int n, hipStream_t s, bool timed, bool capture = false,
+               std::vector<hipEvent_t>* keep_events = nullptr) {
   if (!capture)
     for (int i = 0; i < n; i++) items[i].engine->order_before(s);
   std::vector<nf3d_engine::RangePlan> plans(n);
@@ -1472,6 +1475,9 @@ void run_batch(const nf3d_run_item* items, int n, hipStream_t s, bool timed, boo
   };
   for (int i = 0; i < n; i++) items[i].engine->run_prologue(plans[i], s);
   run_edges(&nf3d_engine::RangePlan::edges_start);
+  // the frame-loop launches, grouped by block shape, mode, bt, sharpen case
+  // and transform form
+  std::vector<std::vector<int>> groups;
   std::vector<bool> done(n, false);
   for (int i = 0; i < n; i++) {
     if (done[i] || !plans[i].run) continue;
@@ -1485,6 +1491,36 @@ void run_batch(const nf3d_run_item* items, int n, hipStream_t s, bool timed, boo
         done[k] = true;
       }
     }
+    groups.push_back(group);
+  }
+  // groups share no engine: with several, each runs on its first engine's
+  // own stream (used by no call but this engine's, which its lock holds
+  // off), forked from `s` and joined back to it, so that the launches of one
+  // group fill the slots another's launch tails leave (in a captured graph:
+  // parallel branches)
+  std::vector<hipEvent_t> joins;
+  hipEvent_t fork = nullptr;
+  struct Events {
+    hipEvent_t& fork;
+    std::vector<hipEvent_t>& joins;
+    std::vector<hipEvent_t>* keep;
+    ~Events() {
+      if (fork) joins.push_back(fork);
+      for (hipEvent_t e : joins) {
+        if (keep) keep->push_back(e);
+        else (void)hipEventDestroy(e);
+      }
+    }
+  } events{fork, joins, keep_events};
+  if (groups.size() > 1) {
+    check(hipEventCreateWithFlags(&fork, hipEventDisableTiming), "hipEventCreate");
+    check(hipEventRecord(fork, s), "hipEventRecord");
+  }
+  for (const std::vector<int>& group : groups) {
+    const int i = group[0];
+    nf3d_engine* e0 = items[i].engine;
+    const hipStream_t gs = groups.size() > 1 ? e0->stream : s;
+    if (gs != s) check(hipStreamWaitEvent(gs, fork, 0), "hipStreamWaitEvent");
     size_t nl = 0;
     for (int k : group) nl = std::max(nl, plans[k].launches.size());
     for (size_t l = 0; l < nl; l++) {
@@ -1504,16 +1540,24 @@ void run_batch(const nf3d_run_item* items, int n, hipStream_t s, bool timed, boo
           B.max_hc = std::max(B.max_hc, e->Hc);
           B.max_frames = std::max(B.max_frames, R.nframes);
         }
-        learned_predictions(items, plans, members, c0, B, plans[i].mode, l, s);
-        for (int c = 0; c < B.njobs; c++) evs.push_back(items[members[c0 + c]].engine->timer(timed, B.job[c].nframes, s));
+        learned_predictions(items, plans, members, c0, B, plans[i].mode, l, gs);
+        for (int c = 0; c < B.njobs; c++)
+          evs.push_back(items[members[c0 + c]].engine->timer(timed, B.job[c].nframes, gs));
         // the seam pass follows on the same stream: run beside the next
         // launch it takes issue slots the latency-bound k_run needs more
-        check(nf3d::launch_run(plans[i].mode, e0->ep.bt, plans[i].shp, e0->dft_mode, e0->ep.bw, e0->ep.bh, B, s),
+        check(nf3d::launch_run(plans[i].mode, e0->ep.bt, plans[i].shp, e0->dft_mode, e0->ep.bw, e0->ep.bh, B, gs),
               "frame-loop kernel");
-        check(nf3d::launch_seam_run(e0->ep.bw, e0->ep.bh, B, s), "seam kernel");
+        check(nf3d::launch_seam_run(e0->ep.bw, e0->ep.bh, B, gs), "seam kernel");
         for (auto& ev : evs)
-          if (ev.second) check(hipEventRecord(ev.second, s), "hipEventRecord");
+          if (ev.second) check(hipEventRecord(ev.second, gs), "hipEventRecord");
       }
+    }
+    if (gs != s) {
+      hipEvent_t j = nullptr;
+      check(hipEventCreateWithFlags(&j, hipEventDisableTiming), "hipEventCreate");
+      joins.push_back(j);
+      check(hipEventRecord(j, gs), "hipEventRecord");
+      check(hipStreamWaitEvent(s, j, 0), "hipStreamWaitEvent");
     }
   }
   run_edges(&nf3d_engine::RangePlan::edges);
@@ -1906,6 +1950,15 @@ int capture_batch(const nf3d_run_item* items, int n, void* stream, nf3d_graph** 
   *graph = nullptr;
   hipStream_t s = static_cast<hipStream_t>(stream);
   bool capturing = false;
+  // fork and join events of the captured batch, destroyed once the capture
+  // has ended (on return)
+  struct CapEvents {
+    std::vector<hipEvent_t> v;
+    ~CapEvents() {
+      for (hipEvent_t e : v) (void)hipEventDestroy(e);
+    }
+  } cap;
+  std::vector<hipEvent_t>& cap_events = cap.v;
   try {
     if (!s) fail("graph capture needs a stream");
     auto locks = lock_batch(items, n);
@@ -1958,7 +2011,7 @@ int capture_batch(const nf3d_run_item* items, int n, void* stream, nf3d_graph** 
     check(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
     capturing = true;
     try {
-      run_batch(items, n, s, false, true);
+      run_batch(items, n, s, false, true, &cap_events);
     } catch (const Failure&) {
       // a capture cut short may have recorded frames in the rings or a kept
       // learned PSD that nothing computed: forget them (the locks still held)
