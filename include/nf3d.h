@@ -111,8 +111,11 @@ int nf3d_engine_run_device(nf3d_engine* e, int first, int count, const void* con
  * its own frame range and planes.  The result is that of calling
  * nf3d_engine_run_device for every item, bit for bit; engines with the same
  * block shape, mode (bt) and sharpen/dehalo case share their frame-loop
- * launches (up to 6 engines per launch), all on `stream` (NULL: the first
- * item's engine's stream).  An engine may appear once per call. */
+ * launches (up to 6 engines per launch), on `stream` (NULL: the first
+ * item's engine's stream); groups of different shapes may run side by side
+ * on their first engines' own streams, forked from `stream` and joined back
+ * to it, so the call's work ends on `stream` (one wait there covers every
+ * item).  An engine may appear once per call. */
 typedef struct {
   nf3d_engine* engine;
   int first, count;
