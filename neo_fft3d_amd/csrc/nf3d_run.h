@@ -535,8 +535,8 @@ __device__ __forceinline__ void split_combine_inv(float2 (&c)[BH / 2], int h, fl
 // workgroups per CU at 48x48): three, or two for the variants whose spectral
 // pass holds more per element (four or five frames, the dehalo factor, the
 // Kalman state with a sharpen pass, 64-row columns), which would otherwise
-// spill; one for the matrix-core form at 48x48 (256 VGPRs and the AGPRs,
-// no spills).
+// spill.  128 lanes (the matrix-core form): two at 32x32, one at 48x48 (256
+// VGPRs and the AGPRs, no spills).
 template <int BH, int NT, int MODE, int BT, int SHP, bool MF>
 constexpr int run_waves_per_simd() {
   if constexpr (MF && BH >= 48) return 1;
