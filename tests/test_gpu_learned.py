@@ -499,6 +499,19 @@ def test_graph_replays_with_learned_psd(torch, period):
         e.close()
 
 
+def test_learned_period_rejects_negative(torch):
+    """nf3d_engine_set_learned_period takes 0 (one prediction per launch) or
+    a frame count; a negative period is refused with the library's message."""
+    from neo_fft3d_amd import Engine, Nf3dError, default_params, make_format
+    from neo_fft3d_amd.learned import set_learned_period
+    e = Engine(default_params(2.0, bt=3, bw=48, bh=48, ow=16, oh=16), make_format("gray", 8), 160, 112, 4, 0)
+    set_learned_period(e, 0)
+    set_learned_period(e, 64)
+    with pytest.raises(Nf3dError, match="prediction period"):
+        set_learned_period(e, -1)
+    e.close()
+
+
 def test_trainer_block_shape_and_sample_scale(torch):
     """A network for 32x32 blocks (overlap 10) in 8-bit sample units: trained
     on the GPU, calibrated on a held-out clip -- the mean of ln t - y over its
