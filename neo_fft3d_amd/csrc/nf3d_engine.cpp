@@ -2138,8 +2138,12 @@ int nf3d_graph_launch(nf3d_graph* g, void* stream) {
       if (f >= 0) f += shift[i];
     e->ring_by_run = g->by_run_after[i];
     e->last_end = g->end_after[i] + shift[i];
+    // the kept span moves with the frames, when they moved by whole spans
     e->lspan = g->lspan_after[i];
-    if (e->lspan >= 0 && e->learned_period > 0) e->lspan += shift[i] / e->learned_period;
+    if (e->lspan >= 0) {
+      const int per = e->learned_period;
+      e->lspan = per > 0 && shift[i] % per == 0 ? e->lspan + shift[i] / per : -1;
+    }
   }
   return 0;
 }
