@@ -499,6 +499,63 @@ def test_graph_replays_with_learned_psd(torch, period):
         e.close()
 
 
+def test_ring_bound_graph_checks_the_kept_span(torch):
+    """A ring-bound graph whose captured launches reuse a learned PSD kept
+    from an earlier call (bt=1: no ring, the span is what it relies on): its
+    replay right after the capture equals one eager call over the same
+    frames bit for bit; once the engine has moved on by a frame count that is
+    not a whole number of spans, the replay is refused and nothing runs."""
+    from neo_fft3d_amd import Engine, Graph, Nf3dError, default_params, make_format, run_device_batch
+    from neo_fft3d_amd.learned import Mlp, noisy_clip, set_learned
+    F, W, H = 12, 160, 112
+    _, noisy, _ = noisy_clip(W, H, F, seed=61)
+    params = rand_params(D, NH, D, seed=8)
+    params[4] = params[4] * 0.3
+    params[5] = np.full(D, np.log((6.0 / 255.0) ** 2 * 48 * 48), dtype=np.float64)
+    m = Mlp(D, NH, D, 64)
+    m.set_params(mo.pack(params).astype(np.float32))
+    m.set_input_norm(np.full(D, -6.0, np.float32), np.full(D, 0.3, np.float32), 1e-6)
+    fmt = make_format("gray", 32, True)
+    src = torch.from_numpy(np.ascontiguousarray(noisy[0])).cuda()
+    st = W * 4
+
+    def engine():
+        e = Engine(default_params(6.0, bt=1, bw=48, bh=48, ow=16, oh=16), fmt, W, H, F, 0)
+        set_learned(e, m, 1.0, period=4)
+        return e
+
+    def item(e, dst, first, count):
+        return (e, first, count, [src[k].data_ptr() for k in range(F)], st, [dst[k].data_ptr() for k in range(F)], st)
+    stream = torch.cuda.Stream()
+    sp = stream.cuda_stream
+    want = torch.zeros_like(src)
+    ref = engine()
+    torch.cuda.synchronize()
+    with torch.cuda.stream(stream):
+        run_device_batch([item(ref, want, 0, 4)], sp)       # frames 0..3: the PSD of frame 0
+    stream.synchronize()
+    got = torch.zeros_like(src)
+    e = engine()
+    with torch.cuda.stream(stream):
+        run_device_batch([item(e, got, 0, 2)], sp)           # span 0 predicted, kept
+        g = Graph([item(e, got, 2, 2)], sp, ring=True)      # frames 2, 3 reuse it
+        g.launch(sp)
+    stream.synchronize()
+    for n in range(4):
+        assert torch.equal(got[n], want[n]), n
+    with torch.cuda.stream(stream):
+        run_device_batch([item(e, got, 4, 5)], sp)           # now 5 frames on: not whole spans
+    stream.synchronize()
+    before = got.cpu().numpy().copy()
+    with pytest.raises(Nf3dError):
+        g.launch(sp)
+    stream.synchronize()
+    assert np.array_equal(got.cpu().numpy(), before)
+    g.close()
+    e.close()
+    ref.close()
+
+
 def test_learned_period_rejects_negative(torch):
     """nf3d_engine_set_learned_period takes 0 (one prediction per launch) or
     a frame count; a negative period is refused with the library's message."""
